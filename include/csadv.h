@@ -1,0 +1,130 @@
+/*
+ * csadv.h -- C ABI of libcsadv.so, the MI355X-native dimension-split finite-volume
+ * advection step on the gnomonic cubed sphere (PPM reconstruction, upwind fluxes,
+ * operator splitting, duo-grid ghost cells).
+ *
+ * Every state buffer is device memory owned by the caller (PyTorch tensors on the
+ * host side); the library keeps non-owning pointers to it and never frees it.  The
+ * library allocates only its own small descriptor tables, in csadv_create; a step
+ * neither allocates nor synchronises the stream.  All entry points return
+ * 0 on success and a negative CSADV_E* code on failure; the message of the last
+ * failure of a handle is returned by csadv_last_error().  Handles are not
+ * thread-safe; independent handles are.
+ *
+ * Layouts (elements; one solve with N cells per panel edge, M = N + 8, halo 4):
+ *   tracer state  q[b] : [ntr][6][M][ld]     i (x) fastest, j (y), panel, tracer
+ *   x-edge winds       : [6][M][ldu]         e = 0..M along x, ldu >= M + 1
+ *   y-edge winds       : [6][M + 1][ld]      e = 0..M along y
+ *   metric  gc / rgc   : [M][ld]   gu : [M][ldu]   gv : [M + 1][ld]  (panel invariant)
+ * The reference stores the same values as numpy arrays indexed [i, j, panel].
+ */
+#ifndef CSADV_H
+#define CSADV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSADV_ABI_VERSION 1
+
+enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
+
+enum csadv_status {
+  CSADV_OK = 0,
+  CSADV_E_ARG = -1,      /* invalid argument / option pair            */
+  CSADV_E_HIP = -2,      /* a HIP runtime call failed                 */
+  CSADV_E_STATE = -3,    /* call out of order (e.g. step before bind) */
+  CSADV_E_RANGE = -4     /* size beyond a documented limit            */
+};
+
+/* Per-solve description.  Option codes are exactly those of par/advection.par
+ * (src/configuration.py:90-158, decoded in src/advection_ic.py:85-149):
+ *   recon 1 PPM-0 | 2 PPM-CW84 | 3 PPM-PL07 | 4 PPM-L04
+ *   dp    1 RK1   | 2 RK2
+ *   split 1 AVLT  | 2 L04      | 3 PL07
+ *   et    3 ET-DG (1 ET-S72 and 2 ET-PL07 are rejected by this build: CSADV_E_ARG)
+ *   mt    1 MT-0  | 2 MT-PL07   (valid pairs: split 1|2 with mt 1, split 3 with mt 2)
+ *   mf    1 none  | 3 MF-PR     (2 MF-AF is rejected by this build: CSADV_E_ARG)   */
+typedef struct csadv_solve_desc {
+  int32_t N, ntr, dtype;
+  int32_t ld, ldu;                 /* row strides (elements) of cell / x-edge arrays   */
+  int32_t recon, dp, split, et, mt, mf;
+  int32_t nbasis;                  /* wind basis fields (>= 1)                          */
+  int32_t unsteady;                /* 1: winds recombined from the basis every step     */
+  int32_t max_steps;               /* rows of the coefficient table (unsteady only)     */
+  double dt, dx, dy;
+  double a2;                       /* sum of gc^2 over the interior of the 6 panels    */
+  /* device buffers */
+  void *q0, *q1;                   /* ping-pong tracer state (dtype)                    */
+  void *uavg, *vavg;               /* time-averaged contravariant winds (dtype)         */
+  void *usgn, *vsgn;               /* contravariant winds fixing the upwind side (dtype)*/
+  const void *gc, *gu, *gv, *rgc;  /* metric tables (dtype)                             */
+  const double *gc64;              /* metric at centres, FP64 (mass fixer, mass)        */
+  const double *lagw;              /* Lagrange weights [4][M][4], FP64                  */
+  const int32_t *lagk;             /* stencil starts   [4][M]                            */
+  const int32_t *orient;           /* halo orientation [6][4][7]                        */
+  double *wbu, *wbv;               /* wind basis, FP64: [nbasis][6][M][ldu] / [nbasis][6][M+1][ld] */
+  const double *conv_pc;           /* [4][6][M][ld]   ex.e_lon, ex.e_lat, ey.e_lon, ey.e_lat       */
+  const double *conv_pu;           /* [5][6][M][ldu]  same four and the determinant at x-edges     */
+  const double *conv_pv;           /* [5][6][M+1][ld] same at y-edges                               */
+  double *wpc;                     /* scratch [nbasis][2][6][M][ld]: lat-lon winds at centres      */
+  const double *coef;              /* [max_steps][2][nbasis] basis coefficients (t_{n-1}, t_{n-2}) */
+  double *corr;                    /* [ntr] pending mass-fixer correction (FP64)        */
+  double *partials;                /* scratch, >= ntr * csadv_partials_per_tracer()     */
+  int32_t *ctl;                    /* [4] device control words: cur, step, 0, 0          */
+} csadv_solve_desc;
+
+typedef struct csadv_s csadv_t;
+
+/* Create a handle for nsolve solves (1 = the reference's single run; > 1 = batched form).
+ * tile selects the step kernel's tile shape (0 = built-in default).
+ * Replaces: adv_simulation_par construction + option validation (src/advection_ic.py:21-203,
+ * src/discrete_operators.py:48-82). */
+int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, csadv_t **out);
+
+/* Number of FP64 partial sums the step writes per tracer (size of desc.partials / ntr). */
+int csadv_partials_per_tracer(int32_t N, int32_t tile);
+
+/* Refresh the device copy of the descriptors after the caller changed pointers. */
+int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, void *stream);
+
+/* Wind preparation from the FP64 basis fields whose interior edges are set:
+ * duo-grid vector ghost fill of every basis field (src/edges_treatment.py:296-347 with
+ * src/interpolation.py:347-532), then the time-averaged winds of step 1
+ * (src/averaged_velocity.py:14-62).  Call once after the basis interiors are written. */
+int csadv_wind_init(csadv_t *h, void *stream);
+
+/* Scalar duo-grid ghost fill of every tracer of the current state buffer
+ * (src/edges_treatment.py:284-290 -> src/interpolation.py:154-315). */
+int csadv_ghost_fill(csadv_t *h, void *stream);
+
+/* Advance every solve nsteps steps: ghost fill, winds (unsteady), split fluxes, update,
+ * mass fixer.  Replaces adv_time_step + update_adv (src/advection_timestep.py:19-75)
+ * called nsteps times.  Graph-capturable (no host synchronisation, no allocation). */
+int csadv_step(csadv_t *h, int32_t nsteps, void *stream);
+
+/* Fold the pending mass-fixer corrections into the current state buffer so that it
+ * holds Q exactly as the reference would after the last step. */
+int csadv_finalize(csadv_t *h, void *stream);
+
+/* Total mass sum_int Q g dx dy of one tracer with a compensated FP64 sum
+ * (src/diagnostics.py:14-26).  Synchronises the stream and writes host out[0]. */
+int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *stream);
+
+/* Error norms of one tracer against qexact (device, [6][N][N] FP64, i fastest):
+ * host out = {Linf, L1, L2} as src/errors.py:99-113 (unweighted means).  Synchronises. */
+int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact, double *out, void *stream);
+
+/* Index (0 or 1) of the buffer holding the current state of a solve (host-tracked). */
+int csadv_current(csadv_t *h, int32_t solve);
+
+const char *csadv_last_error(csadv_t *h);
+void csadv_destroy(csadv_t *h);
+int csadv_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSADV_H */

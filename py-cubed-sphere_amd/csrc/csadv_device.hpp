@@ -1,0 +1,142 @@
+// Device-side helpers shared by the csadv kernels: descriptor layout, PPM edge
+// reconstructions and the upwind flux.  gfx950 (CDNA4) only; 64-wide wavefronts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "csadv.h"
+
+namespace csadv {
+
+constexpr int NG = 4;          // ghost layers per side (src/cs_datastruct.py:225-231)
+constexpr int BLOCK = 256;     // threads per block of the step kernel (4 waves)
+
+// Device copy of one solve: the public descriptor plus derived strides.
+struct Solve {
+  csadv_solve_desc d;
+  int32_t M;
+  int32_t tiles_x, tiles_y, tiles;   // step tiling of one panel
+  int64_t pstride, tstride;          // cell arrays: panel / tracer strides
+  int64_t pstride_u, pstride_v;      // x-edge / y-edge arrays: panel strides
+  int32_t part_per_tracer;           // partial sums per tracer (= 6 * tiles)
+  int32_t pad;
+};
+
+// A set of FP64 or dtype fields that receive a duo-grid ghost fill.
+struct FieldSet {
+  void *base[2];
+  const int32_t *cur;        // device current-buffer index, or null (base[0])
+  int32_t nfield, N, M, ld;
+  int64_t fstride, pstride;
+  const double *corr;        // per-field pending correction (null = none)
+  const double *gc64;        // [M][ld] metric, used with corr
+  const double *lagw;
+  const int32_t *lagk;
+  const int32_t *orient;
+};
+
+// Neighbour cell of the halo strip (panel p, side s) at depth d and along-edge index r.
+// orient row = {q, ib, id, ir, jb, jd, jr}: i = ib + id*d + ir*r, j = jb + jd*d + jr*r.
+__device__ __forceinline__ void halo_src(const int32_t *o, int d, int r, int &q, int &i, int &j) {
+  q = o[0];
+  i = o[1] + o[2] * d + o[3] * r;
+  j = o[4] + o[5] * d + o[6] * r;
+}
+
+// Ghost target of (side, depth d, along-edge k); sides 0 E, 1 W, 2 N, 3 S.
+__device__ __forceinline__ void ghost_target(int side, int d, int k, int N, int &i, int &j) {
+  const int iend = N + NG;
+  switch (side) {
+    case 0: i = iend + d; j = k; break;
+    case 1: i = NG - 1 - d; j = k; break;
+    case 2: i = k; j = iend + d; break;
+    default: i = k; j = NG - 1 - d; break;
+  }
+}
+
+template <typename T> __device__ __forceinline__ T sgn(T x) { return (x > T(0)) ? T(1) : ((x < T(0)) ? T(-1) : T(0)); }
+
+// PPM edge values (q_L, q_R) of one cell from its 5-cell window; src/reconstruction_1d.py:26-193.
+//   RECON 1 PPM-0 (CW84 eq. 1.9), 2 PPM-CW84 (monotone), 3 PPM-PL07 (eqs. 41-42), 4 PPM-L04 (B1-B4)
+template <typename T, int RECON>
+__device__ __forceinline__ void recon_edges(T m2, T m1, T c0, T p1, T p2, T &qL, T &qR) {
+  if constexpr (RECON == 3) {
+    const T a1 = T(2.0 / 60.0), a2 = T(-13.0 / 60.0), a3 = T(47.0 / 60.0), a4 = T(27.0 / 60.0), a5 = T(-3.0 / 60.0);
+    qR = a1 * m2 + a2 * m1 + a3 * c0 + a4 * p1 + a5 * p2;
+    qL = a5 * m2 + a4 * m1 + a3 * c0 + a2 * p1 + a1 * p2;
+  } else if constexpr (RECON == 1) {
+    const T s7 = T(7.0 / 12.0);
+    qR = s7 * (p1 + c0) - (p2 + m1) / T(12.0);
+    qL = s7 * (c0 + m1) - (p1 + m2) / T(12.0);
+  } else if constexpr (RECON == 2) {
+    // limited slopes at c-1, c, c+1 (CW84 eqs. 1.7-1.8)
+    auto slope = [](T a, T b, T c) {
+      T d0 = T(0.5) * (c - a), d1 = T(2.0) * (c - b), d2 = T(2.0) * (b - a);
+      T d = fmin(fabs(d0), fabs(d1));
+      d = fmin(d, fabs(d2)) * sgn(d0);
+      return ((c - b) * (b - a) > T(0)) ? d : T(0);
+    };
+    T sm = slope(m2, m1, c0), s0 = slope(m1, c0, p1), sp = slope(c0, p1, p2);
+    T R = T(0.5) * (p1 + c0) - (sp - s0) / T(6.0);
+    T L = T(0.5) * (c0 + m1) - (s0 - sm) / T(6.0);
+    T dq = R - L, q6 = T(6) * c0 - T(3) * (R + L);
+    if ((R - c0) * (c0 - L) <= T(0)) { R = c0; L = c0; }     // local extremum: flatten (eq. 1.10)
+    bool over = fabs(dq) < fabs(q6);
+    T dd = R - L, mm = c0 - T(0.5) * (R + L);
+    bool left = dd * mm > (dd * dd) / T(6.0);
+    bool right = -(dd * dd) / T(6.0) > dd * mm;
+    if (over && left) L = T(3.0) * c0 - T(2.0) * R;
+    if (over && right) R = T(3.0) * c0 - T(2.0) * L;
+    qL = L; qR = R;
+  } else {
+    // monotone slope (L04 eq. B1) at c-1, c, c+1
+    auto mono = [](T a, T b, T c) {
+      T d = T(0.25) * (c - a);
+      T mx = fmax(fmax(a, b), c), mn = fmin(fmin(a, b), c);
+      return fmin(fmin(fabs(d), mx - b), b - mn) * sgn(d);
+    };
+    T dm = mono(m2, m1, c0), d0 = mono(m1, c0, p1), dp = mono(c0, p1, p2);
+    T R = T(0.5) * (p1 + c0) - (dp - d0) / T(3.0);
+    T L = T(0.5) * (c0 + m1) - (d0 - dm) / T(3.0);
+    T s = sgn(T(2.0) * d0);
+    qL = c0 - fmin(T(2.0) * fabs(d0), fabs(L - c0)) * s;   // eq. B3
+    qR = c0 + fmin(T(2.0) * fabs(d0), fabs(R - c0)) * s;   // eq. B4
+  }
+}
+
+// Upwind PPM flux through one edge (src/flux.py:20-70).  w points at the upwind cell in a
+// row/column with element stride s; upos selects the left cell (flux from the left).
+// cc = Courant number, uav = time-averaged wind, gl / gr / gcell = metric at the cell's
+// left edge, right edge and centre, ge = metric at the flux edge (MT-PL07).
+template <typename T, int RECON>
+__device__ __forceinline__ T upwind_flux(const T *w, int s, bool upos, T cc, T uav, T gl, T gr, T gcell,
+                                         bool mt0, T ge) {
+  T qL, qR;
+  recon_edges<T, RECON>(w[-2 * s], w[-s], w[0], w[s], w[2 * s], qL, qR);
+  T q = w[0];
+  if (mt0) { qL = qL * gl; qR = qR * gr; q = q * gcell; }
+  const T dq = qR - qL;
+  const T q6 = T(6) * q - T(3) * (qR + qL);
+  const T third = T(1.0 / 3.0);
+  T f = upos ? (qR + (cc * T(0.5)) * (q6 - dq) - ((q6 * cc) * cc) * third)
+             : (qL - (cc * T(0.5)) * (q6 + dq) - ((q6 * cc) * cc) * third);
+  f = f * uav;
+  if (!mt0) f = f * ge;
+  return f;
+}
+
+// Split update of one cell after the first directional pass (src/discrete_operators.py:49-73).
+//   split 1 AVLT: (gQ + dF/2)/g ; 2 L04: (gQ + dF/2 + (c1-c2) Q/2)/g ; 3 PL07: (Q + (Q+dF)/(1-(c1-c2)))/2
+template <typename T>
+__device__ __forceinline__ T split_update(int split, T Q, T dF, T g, T rg, T c1, T c2) {
+  if (split == 1) return (Q * g + T(0.5) * dF) * rg;
+  if (split == 2) return (Q * g + T(0.5) * dF + T(0.5) * (c1 - c2) * Q) * rg;
+  return T(0.5) * (Q + (Q + dF) / (T(1.0) - (c1 - c2)));
+}
+
+__device__ __forceinline__ double warp_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+}  // namespace csadv

@@ -1,0 +1,123 @@
+"""FP64 geometry of the extended gnomonic cubed sphere, built on the host once per N.
+
+Everything here is init-time host work: the step itself runs in libcsadv.  Arrays are
+produced directly in the device layout ``[panel][j][i]`` (i = x fastest).
+
+Design: every panel is the panel-0 construction rotated by a signed permutation
+matrix (``PANEL_ROT``), so positions, tangent vectors and lat-lon unit vectors come
+from one code path.  Positions reproduce src/cs_transform.py:41-94 bit for bit (a
+signed permutation is exact); tangent vectors and the metric use the closed forms
+
+    e_x = (1+tx^2)/D^3 (-tx, 1+ty^2, -tx ty),   e_y = (1+ty^2)/D^3 (-ty, -tx ty, 1+tx^2),
+    g   = (1+tx^2)(1+ty^2)/D^3,                D = sqrt(1 + tx^2 + ty^2),
+
+which agree with the reference's construction (src/cs_transform.py:245-422,
+src/cs_datastruct.py:404-493) to rounding.
+"""
+import math
+
+import numpy as np
+
+NG = 4
+PIO4 = math.pi / 4.0
+
+# world = R_p @ (c0, c1, c2) with c = (1, tan x, tan y)/D  (src/cs_transform.py:62-91)
+PANEL_ROT = np.array([
+    [[1, 0, 0], [0, 1, 0], [0, 0, 1]],
+    [[0, -1, 0], [1, 0, 0], [0, 0, 1]],
+    [[-1, 0, 0], [0, -1, 0], [0, 0, 1]],
+    [[0, 1, 0], [-1, 0, 0], [0, 0, 1]],
+    [[0, 0, -1], [0, 1, 0], [1, 0, 0]],
+    [[0, 0, 1], [0, 1, 0], [-1, 0, 0]],
+], dtype=np.int64)
+
+
+def _rotate(p, c0, c1, c2):
+    """Apply the signed permutation of panel p exactly (no multiplications by +-1 rounding)."""
+    comps = (c0, c1, c2)
+    out = []
+    for row in PANEL_ROT[p]:
+        k = int(np.nonzero(row)[0][0])
+        out.append(comps[k] if row[k] > 0 else -comps[k])
+    return out
+
+
+class PointSet:
+    """Geometry at one family of points (centres, x-edges or y-edges), layout [6][ny][nx]."""
+
+    def __init__(self, x, y):
+        # x varies along the last axis, y along the middle axis
+        yy, xx = np.meshgrid(y, x, indexing="ij")
+        tx, ty = np.tan(xx), np.tan(yy)
+        D2 = 1.0 + tx**2 + ty**2
+        D = np.sqrt(D2)
+        invD = 1.0 / D
+        c0, c1, c2 = invD, invD * tx, invD * ty
+        D3 = D2 * D
+        fx = (1.0 + tx * tx) / D3
+        fy = (1.0 + ty * ty) / D3
+        ex0 = (-tx * fx, (1.0 + ty * ty) * fx, -tx * ty * fx)
+        ey0 = (-ty * fy, -tx * ty * fy, (1.0 + tx * tx) * fy)
+        self.g = (1.0 + tx * tx) * (1.0 + ty * ty) / D3
+        shape = (6,) + xx.shape
+        self.X, self.Y, self.Z = np.empty(shape), np.empty(shape), np.empty(shape)
+        self.lon, self.lat = np.empty(shape), np.empty(shape)
+        self.ex_lon, self.ex_lat = np.empty(shape), np.empty(shape)
+        self.ey_lon, self.ey_lat = np.empty(shape), np.empty(shape)
+        for p in range(6):
+            X, Y, Z = _rotate(p, c0, c1, c2)
+            ex = _rotate(p, *ex0)
+            ey = _rotate(p, *ey0)
+            lon = np.arctan2(Y, X)
+            lat = np.arctan2(Z, np.hypot(X, Y))
+            sl, cl, st, ct = np.sin(lon), np.cos(lon), np.sin(lat), np.cos(lat)
+            elon = (-sl, cl, 0.0)
+            elat = (-st * cl, -st * sl, ct)
+            self.X[p], self.Y[p], self.Z[p], self.lon[p], self.lat[p] = X, Y, Z, lon, lat
+            self.ex_lon[p] = ex[0] * elon[0] + ex[1] * elon[1]
+            self.ex_lat[p] = ex[0] * elat[0] + ex[1] * elat[1] + ex[2] * elat[2]
+            self.ey_lon[p] = ey[0] * elon[0] + ey[1] * elon[1]
+            self.ey_lat[p] = ey[0] * elat[0] + ey[1] * elat[1] + ey[2] * elat[2]
+        self.det = self.ex_lon * self.ey_lat - self.ey_lon * self.ex_lat
+
+
+class Grid:
+    """Equiangular gnomonic cubed sphere with 4 ghost layers (src/cs_datastruct.py:195-541).
+
+    Attributes: N, M = N+8, dx = dy = (pi/2)/N, i0 = j0 = 4, iend = jend = N+4, point sets
+    ``pc`` (centres, [6][M][M]), ``pu`` (x-edges, [6][M][M+1]), ``pv`` (y-edges, [6][M+1][M]).
+    """
+
+    projection = "gnomonic_equiangular"
+
+    def __init__(self, N):
+        if N < 8:
+            raise ValueError("N must be >= 8 (the 4-cell boundary frames of opposite edges may not overlap)")
+        self.N = N
+        self.M = N + 2 * NG
+        self.ng, self.ngl, self.ngr = 2 * NG, NG, NG
+        a = PIO4
+        self.a = a
+        dx = (a - (-a)) / N
+        self.dx = self.dy = dx
+        self.i0 = self.j0 = NG
+        self.iend = self.jend = N + NG
+        # same 1-D coordinates as src/cs_datastruct.py:272-273, 298-300
+        self.x_pc = np.linspace(-a + dx / 2.0 - NG * dx, a - dx / 2.0 + NG * dx, N + 2 * NG)
+        self.x_pu = np.linspace(-a - NG * dx, a + NG * dx, N + 1 + 2 * NG)
+        self.pc = PointSet(self.x_pc, self.x_pc)
+        self.pu = PointSet(self.x_pu, self.x_pc)
+        self.pv = PointSet(self.x_pc, self.x_pu)
+        I = slice(self.i0, self.iend)
+        gi = self.pc.g[I, I]
+        self.a2 = 6.0 * float(np.sum(gi * gi))   # sum of g^2 over the 6 panel interiors
+
+    # helpers for the reference layout Q[i, j, p]
+    @staticmethod
+    def to_reference(a):
+        """[..., 6, ny, nx] -> [nx, ny, 6] (reference numpy layout)."""
+        return np.ascontiguousarray(np.moveaxis(np.asarray(a), (-3, -2, -1), (-1, -2, -3)))
+
+    @staticmethod
+    def from_reference(a):
+        return np.ascontiguousarray(np.moveaxis(np.asarray(a), (-1, -2, -3), (-3, -2, -1)))

@@ -1,0 +1,266 @@
+"""Device solver: torch-owned buffers plus libcsadv through its C ABI.
+
+``Advection`` is the drop-in for the reference's ``adv_simulation_par`` + ``init_vars_adv``
++ ``adv_time_step`` / ``update_adv`` loop (src/advection_ic.py:21-203,
+src/advection_vars.py:19-107, src/advection_timestep.py:19-75).  The host builds the
+FP64 tables once (grid, metric, duo-grid weights, wind basis, initial condition), the
+device does everything per step.  There is no CPU fallback: without libcsadv.so or a
+GPU every call raises.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .grid import Grid
+from .topology import lagrange_tables, orientation
+
+_TDT = {"f64": torch.float64, "f32": torch.float32}
+
+
+def _round_up(x, m):
+    return (x + m - 1) // m * m
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _pad_last(a, width):
+    """Zero-pad the last axis of a numpy array to ``width`` columns."""
+    a = np.asarray(a)
+    out = np.zeros(a.shape[:-1] + (width,), dtype=a.dtype)
+    out[..., :a.shape[-1]] = a
+    return out
+
+
+class GridTables:
+    """Device copies of the per-N tables (shared by every solve of that N and dtype)."""
+
+    def __init__(self, grid, dtype, device):
+        self.grid = grid
+        N, M = grid.N, grid.M
+        self.ld = M if M % 4 == 0 else _round_up(M, 4)
+        self.ldu = _round_up(M + 1, 4)
+        td = _TDT[dtype]
+        dev = device
+        ld, ldu = self.ld, self.ldu
+
+        def up(a, tdt=torch.float64):
+            return torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=tdt)
+
+        g0 = grid.pc.g
+        self.gc64 = up(_pad_last(g0, ld))
+        self.gc = up(_pad_last(g0, ld), td)
+        self.rgc = up(_pad_last(1.0 / g0, ld), td)
+        self.gu = up(_pad_last(grid.pu.g, ldu), td)
+        self.gv = up(_pad_last(grid.pv.g, ld), td)
+        w, k = lagrange_tables(grid)
+        self.lagw = up(w)
+        self.lagk = up(k, torch.int32)
+        self.orient = up(orientation(N), torch.int32)
+        pc, pu, pv = grid.pc, grid.pu, grid.pv
+        self.conv_pc = up(_pad_last(np.stack([pc.ex_lon, pc.ex_lat, pc.ey_lon, pc.ey_lat]), ld))
+        self.conv_pu = up(_pad_last(np.stack([pu.ex_lon, pu.ex_lat, pu.ey_lon, pu.ey_lat, pu.det]), ldu))
+        self.conv_pv = up(_pad_last(np.stack([pv.ex_lon, pv.ex_lat, pv.ey_lon, pv.ey_lat, pv.det]), ld))
+
+
+def interior_winds(grid, case):
+    """Contravariant basis winds at the interior edges (src/advection_vars.py:37-53):
+    returns (wu [K][6][M][M+1], wv [K][6][M+1][M]) with zeros outside the interior."""
+    i0, iend = grid.i0, grid.iend
+    M = grid.M
+    basis = case.basis()
+    K = len(basis)
+    wu = np.zeros((K, 6, M, M + 1))
+    wv = np.zeros((K, 6, M + 1, M))
+    su = (slice(None), slice(i0, iend), slice(i0, iend + 1))
+    sv = (slice(None), slice(i0, iend + 1), slice(i0, iend))
+    pu, pv = grid.pu, grid.pv
+    for b, (fu, fv) in enumerate(basis):
+        ul, vl = fu(pu.lon[su], pu.lat[su]), fv(pu.lon[su], pu.lat[su])
+        wu[b][su] = (pu.ey_lat[su] * ul - pu.ey_lon[su] * vl) / pu.det[su]      # src/sphgeo.py:120-125
+        ul, vl = fu(pv.lon[sv], pv.lat[sv]), fv(pv.lon[sv], pv.lat[sv])
+        wv[b][sv] = (-pv.ex_lat[sv] * ul + pv.ex_lon[sv] * vl) / pv.det[sv]
+    return wu, wv
+
+
+def cfl_dt(grid, case, cfl):
+    """Largest dt with max |contravariant wind| * dt / dx <= cfl over the interior edges at t=0."""
+    wu, wv = interior_winds(grid, case)
+    s = case.coefficients(0.0)
+    u = np.tensordot(s, wu, axes=1)
+    v = np.tensordot(s, wv, axes=1)
+    vmax = max(np.abs(u).max(), np.abs(v).max())
+    return cfl * grid.dx / vmax
+
+
+class Advection:
+    """One solve on the device.
+
+    Parameters mirror ``adv_simulation_par`` (src/advection_ic.py:21): ``dt``, option
+    codes ``recon, dp, opsplit, et, mt, mf`` from par/advection.par, and a
+    :class:`cubedsphere.cases.Case`.  ``dtype`` is "f64" (parity mode) or "f32".
+    """
+
+    def __init__(self, N, case, dt, recon=3, dp=1, opsplit=1, et=3, mt=1, mf=3, dtype="f64",
+                 device="cuda", tile=0, max_steps=1, tables=None, grid=None):
+        if not torch.cuda.is_available():
+            raise _lib.CsadvError("libcsadv needs a GPU; no CPU fallback exists")
+        self.lib = _lib.load()
+        self.grid = grid if grid is not None else (tables.grid if tables is not None else Grid(N))
+        self.case = case
+        self.dt = float(dt)
+        self.opts = dict(recon=recon, dp=dp, split=opsplit, et=et, mt=mt, mf=mf)
+        self.dtype = dtype
+        self.device = torch.device(device)
+        self.tile = tile
+        self.tab = tables if tables is not None else GridTables(self.grid, dtype, self.device)
+        self.max_steps = max(1, int(max_steps))
+        self.handle = None
+        self.steps_done = 0
+        self._alloc()
+
+    # ------------------------------------------------------------ buffers
+    def _alloc(self):
+        g, tab, dev = self.grid, self.tab, self.device
+        N, M, ld, ldu = g.N, g.M, tab.ld, tab.ldu
+        td = _TDT[self.dtype]
+        ntr = self.case.ntr
+        K = len(self.case.basis())
+        f64 = dict(device=dev, dtype=torch.float64)
+        self.q = [torch.zeros((ntr, 6, M, ld), device=dev, dtype=td) for _ in range(2)]
+        self.uavg = torch.zeros((6, M, ldu), device=dev, dtype=td)
+        self.vavg = torch.zeros((6, M + 1, ld), device=dev, dtype=td)
+        # RK2 picks the upwind side from the un-averaged wind (src/averaged_velocity.py:21-27)
+        rk2 = self.opts["dp"] == 2
+        self.usgn = torch.zeros_like(self.uavg) if rk2 else None
+        self.vsgn = torch.zeros_like(self.vavg) if rk2 else None
+        self.wbu = torch.zeros((K, 6, M, ldu), **f64)
+        self.wbv = torch.zeros((K, 6, M + 1, ld), **f64)
+        self.wpc = torch.zeros((K, 2, 6, M, ld), **f64)
+        steps = 1 if self.case.steady else self.max_steps
+        self.coef = torch.from_numpy(self.case.coefficient_table(steps, self.dt).reshape(-1)).to(**f64)
+        self.corr = torch.zeros(ntr, **f64)
+        nparts = self.lib.csadv_partials_per_tracer(N, self.tile)
+        _lib.check(nparts, None, "csadv_partials_per_tracer")
+        self.partials = torch.zeros(ntr * nparts, **f64)
+        self.ctl = torch.zeros(4, device=dev, dtype=torch.int32)
+
+    def desc(self):
+        g, tab = self.grid, self.tab
+        d = _lib.SolveDesc()
+        d.N, d.ntr = g.N, self.case.ntr
+        d.dtype = _lib.CSADV_F64 if self.dtype == "f64" else _lib.CSADV_F32
+        d.ld, d.ldu = tab.ld, tab.ldu
+        for k, v in self.opts.items():
+            setattr(d, k, v)
+        d.nbasis = len(self.case.basis())
+        d.unsteady = 0 if self.case.steady else 1
+        d.max_steps = 1 if self.case.steady else self.max_steps
+        d.dt, d.dx, d.dy, d.a2 = self.dt, g.dx, g.dy, g.a2
+        d.q0, d.q1 = _ptr(self.q[0]), _ptr(self.q[1])
+        d.uavg, d.vavg, d.usgn, d.vsgn = _ptr(self.uavg), _ptr(self.vavg), _ptr(self.usgn), _ptr(self.vsgn)
+        d.gc, d.gu, d.gv, d.rgc, d.gc64 = _ptr(tab.gc), _ptr(tab.gu), _ptr(tab.gv), _ptr(tab.rgc), _ptr(tab.gc64)
+        d.lagw, d.lagk, d.orient = _ptr(tab.lagw), _ptr(tab.lagk), _ptr(tab.orient)
+        d.wbu, d.wbv = _ptr(self.wbu), _ptr(self.wbv)
+        d.conv_pc, d.conv_pu, d.conv_pv, d.wpc = _ptr(tab.conv_pc), _ptr(tab.conv_pu), _ptr(tab.conv_pv), _ptr(self.wpc)
+        d.coef, d.corr, d.partials, d.ctl = _ptr(self.coef), _ptr(self.corr), _ptr(self.partials), _ptr(self.ctl)
+        return d
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    # ------------------------------------------------------------ init (src/advection_vars.py:19-107)
+    def init_vars_adv(self):
+        g = self.grid
+        i0, iend = g.i0, g.iend
+        wu, wv = interior_winds(g, self.case)
+        ldu, ld = self.tab.ldu, self.tab.ld
+        self.wbu.copy_(torch.from_numpy(_pad_last(wu, ldu)))
+        self.wbv.copy_(torch.from_numpy(_pad_last(wv, ld)))
+        I = slice(i0, iend)
+        q0 = np.stack([self.case.q(n, g.pc.lon[:, I, I], g.pc.lat[:, I, I], 0.0) for n in range(self.case.ntr)])
+        self.q[0].zero_()
+        self.q[0][:, :, I, I] = torch.from_numpy(q0).to(self.q[0].dtype)
+        self.q[1].zero_()
+        self.ctl.zero_()
+        self.corr.zero_()
+        torch.cuda.synchronize(self.device)
+        h = ctypes.c_void_p()
+        d = self.desc()
+        rc = self.lib.csadv_create(ctypes.byref(d), 1, self.tile, ctypes.byref(h))
+        self.handle = h
+        _lib.check(rc, h, "csadv_create")
+        _lib.check(self.lib.csadv_wind_init(h, self._stream()), h, "csadv_wind_init")
+        self.steps_done = 0
+        return self
+
+    # ------------------------------------------------------------ stepping
+    def step(self, nsteps=1):
+        """nsteps calls of adv_time_step + update_adv (src/advection_timestep.py:19-75)."""
+        if self.handle is None:
+            raise _lib.CsadvError("init_vars_adv() first")
+        if not self.case.steady and self.steps_done + nsteps > self.max_steps:
+            raise _lib.CsadvError("unsteady winds: coefficient table holds %d steps" % self.max_steps)
+        _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
+        self.steps_done += nsteps
+
+    def ghost_fill(self):
+        """Duo-grid ghost fill of the current state (src/edges_treatment.py:284-290)."""
+        _lib.check(self.lib.csadv_ghost_fill(self.handle, self._stream()), self.handle, "csadv_ghost_fill")
+
+    def winds(self):
+        """Time-averaged winds of the next step: (uavg [6][M][M+1], vavg [6][M+1][M]) numpy."""
+        M = self.grid.M
+        return self.uavg[:, :, :M + 1].double().cpu().numpy(), self.vavg[:, :, :M].double().cpu().numpy()
+
+    def current(self):
+        return _lib.check(self.lib.csadv_current(self.handle, 0), self.handle, "csadv_current")
+
+    def finalize(self):
+        _lib.check(self.lib.csadv_finalize(self.handle, self._stream()), self.handle, "csadv_finalize")
+
+    def state(self):
+        """Interior of every tracer after folding in pending corrections: [ntr][6][N][N] FP64 numpy."""
+        self.finalize()
+        g = self.grid
+        I = slice(g.i0, g.iend)
+        return self.q[self.current()][:, :, I, I].double().cpu().numpy()
+
+    def full_state(self):
+        """Every tracer including ghosts: [ntr][6][M][M] numpy (dtype of the solve)."""
+        self.finalize()
+        M = self.grid.M
+        return self.q[self.current()][:, :, :, :M].cpu().numpy()
+
+    def mass(self, tracer=0):
+        out = ctypes.c_double()
+        _lib.check(self.lib.csadv_mass(self.handle, 0, tracer, ctypes.byref(out), self._stream()), self.handle, "csadv_mass")
+        return out.value
+
+    def exact(self, tracer=0, t=None):
+        g = self.grid
+        t = self.steps_done * self.dt if t is None else t
+        I = slice(g.i0, g.iend)
+        return self.case.q(tracer, g.pc.lon[:, I, I], g.pc.lat[:, I, I], t)
+
+    def norms(self, tracer=0, t=None):
+        """(Linf, L1, L2) against the exact solution (src/errors.py:99-113)."""
+        qe = torch.from_numpy(np.ascontiguousarray(self.exact(tracer, t))).to(self.device, torch.float64)
+        out = (ctypes.c_double * 3)()
+        _lib.check(self.lib.csadv_norms(self.handle, 0, tracer, _ptr(qe), out, self._stream()), self.handle, "csadv_norms")
+        return tuple(out)
+
+    def close(self):
+        if self.handle:
+            self.lib.csadv_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
