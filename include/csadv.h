@@ -120,6 +120,13 @@ int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact,
 /* Index (0 or 1) of the buffer holding the current state of a solve (host-tracked). */
 int csadv_current(csadv_t *h, int32_t solve);
 
+/* Measurement hook: capacity > 0 creates that many HIP event pairs and makes csadv_step
+ * record one pair around each launch of the step kernel (until the pairs run out);
+ * capacity = 0 releases them.  csadv_profile_read waits for the recorded pairs and
+ * returns the summed kernel time (ms) and the number of launches, then resets. */
+int csadv_profile(csadv_t *h, int32_t capacity);
+int csadv_profile_read(csadv_t *h, double *total_ms, int32_t *launches);
+
 const char *csadv_last_error(csadv_t *h);
 void csadv_destroy(csadv_t *h);
 int csadv_abi_version(void);

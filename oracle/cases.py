@@ -230,3 +230,15 @@ def case_q(case, tracer, lon, lat, t):
         return field_xyz(name, Xr, Yr, Zr, params)
     # deformational and other time-reversing flows return to the IC at t = period
     return field(name, lon, lat, params)
+
+
+def from_case(case):
+    """Case dict for an object with the product's ``Case`` attributes (``wind``, ``tracers``)."""
+    kind = case.wind[0]
+    if kind == "ref":
+        if case.ntr == 1 and isinstance(case.tracers[0], int):
+            return {"ic": case.tracers[0], "vf": case.wind[1]}
+        return {"vf": case.wind[1], "tracers": list(case.tracers)}
+    if kind == "solid_body":
+        return {"vf": "solid_body", "alpha": case.wind[1], "period": case.wind[2], "tracers": list(case.tracers)}
+    return {"vf": "deformational", "k": case.wind[1], "period": case.wind[2], "tracers": list(case.tracers)}

@@ -605,6 +605,8 @@ struct csadv_s {
   int recon = 3;
   double *red = nullptr;          // small reduction scratch (mass / norms)
   std::vector<int> cur;           // host copy of the current-buffer index per solve
+  std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
+  int ev_used = 0;
   std::string err;
 };
 
@@ -877,8 +879,14 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream) {
     if (rc) return rc;
     rc = wind_avg(h, st, false);
     if (rc) return rc;
+    const bool prof = 2 * (h->ev_used + 1) <= (int)h->ev.size();
+    if (prof) HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used], st));
     hipError_t e = h->dtype == CSADV_F64 ? launch_step_dt<double>(h, st) : launch_step_dt<float>(h, st);
     if (e != hipSuccess) return fail(h, CSADV_E_HIP, std::string("step kernel: ") + hipGetErrorString(e));
+    if (prof) {
+      HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used + 1], st));
+      ++h->ev_used;
+    }
     hipLaunchKernelGGL(k_step_reduce, dim3(ns), dim3(BLOCK), 0, st, h->dev);
     HIPCHK(h, hipGetLastError());
   }
@@ -950,6 +958,35 @@ int csadv_current(csadv_t *h, int32_t solve) {
   return h->cur[solve];
 }
 
+int csadv_profile(csadv_t *h, int32_t capacity) {
+  if (!h) return CSADV_E_ARG;
+  if (capacity < 0 || capacity > (1 << 16)) return fail(h, CSADV_E_RANGE, "profile capacity");
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  h->ev.clear();
+  h->ev_used = 0;
+  for (int i = 0; i < 2 * capacity; ++i) {
+    hipEvent_t e;
+    HIPCHK(h, hipEventCreate(&e));
+    h->ev.push_back(e);
+  }
+  return CSADV_OK;
+}
+
+int csadv_profile_read(csadv_t *h, double *total_ms, int32_t *launches) {
+  if (!h || !total_ms || !launches) return CSADV_E_ARG;
+  double tot = 0.0;
+  for (int k = 0; k < h->ev_used; ++k) {
+    HIPCHK(h, hipEventSynchronize(h->ev[2 * k + 1]));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = h->ev_used;
+  h->ev_used = 0;
+  return CSADV_OK;
+}
+
 const char *csadv_last_error(csadv_t *h) { return h ? h->err.c_str() : "null handle"; }
 
 void csadv_destroy(csadv_t *h) {
@@ -959,6 +996,7 @@ void csadv_destroy(csadv_t *h) {
   if (h->qsets) (void)hipFree(h->qsets);
   if (h->wsets) (void)hipFree(h->wsets);
   if (h->red) (void)hipFree(h->red);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   delete h;
 }
 

@@ -254,6 +254,17 @@ class Advection:
         _lib.check(self.lib.csadv_norms(self.handle, 0, tracer, _ptr(qe), out, self._stream()), self.handle, "csadv_norms")
         return tuple(out)
 
+    def profile(self, capacity):
+        """Record HIP events around each step-kernel launch of the next ``capacity`` steps."""
+        _lib.check(self.lib.csadv_profile(self.handle, int(capacity)), self.handle, "csadv_profile")
+
+    def profile_read(self):
+        """(summed step-kernel milliseconds, launches) since the last read."""
+        ms, n = ctypes.c_double(), ctypes.c_int32()
+        _lib.check(self.lib.csadv_profile_read(self.handle, ctypes.byref(ms), ctypes.byref(n)), self.handle,
+                   "csadv_profile_read")
+        return ms.value, n.value
+
     def close(self):
         if self.handle:
             self.lib.csadv_destroy(self.handle)

@@ -4,6 +4,7 @@ import math
 import numpy as np
 
 from oracle import timestep as ots
+from oracle.cases import from_case
 
 PI = math.pi
 
@@ -15,14 +16,7 @@ def ref_to_dev(a):
 
 def oracle_case(case):
     """Translate a product Case into the oracle's case dict."""
-    kind = case.wind[0]
-    if kind == "ref":
-        if all(isinstance(t, int) for t in case.tracers) and case.ntr == 1:
-            return {"ic": case.tracers[0], "vf": case.wind[1]}
-        return {"vf": case.wind[1], "tracers": case.tracers}
-    if kind == "solid_body":
-        return {"vf": "solid_body", "alpha": case.wind[1], "period": case.wind[2], "tracers": case.tracers}
-    return {"vf": "deformational", "k": case.wind[1], "period": case.wind[2], "tracers": case.tracers}
+    return from_case(case)
 
 
 def run_oracle(N, case, dt, nsteps, **opts):

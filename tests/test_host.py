@@ -1,0 +1,207 @@
+"""Host-side product code on the CPU: FP64 tables, topology, problem definitions, par
+readers and the C ABI's exported symbols (no device calls)."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cubedsphere import config
+from cubedsphere.cases import Case, reference_case, workload, random_solid_body
+from cubedsphere.grid import Grid
+from cubedsphere.topology import halo_source, lagrange_tables, orientation
+from oracle import cases as ocases
+from oracle import grid as ogrid
+from oracle import lagrange as olagrange
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = np.load(os.path.join(ROOT, "tests", "golden", "ref_lagrange_halo.npz"))
+R = Grid.to_reference
+
+
+def _b6(a):
+    return np.broadcast_to(a, (6,) + a.shape)
+
+
+@pytest.mark.parametrize("N", [8, 16, 40])
+def test_grid_tables_match_oracle(N):
+    g, o = Grid(N), ogrid.CubedSphere(N)
+    assert np.array_equal(R(g.pc.X), o.pc.X) and np.array_equal(R(g.pu.Z), o.pu.Z)
+    assert np.array_equal(R(g.pc.lon), o.pc.lon) and np.array_equal(R(g.pv.lat), o.pv.lat)
+    for mine, ref in ((g.pc.g, o.metric_tensor_pc), (g.pu.g, o.metric_tensor_pu), (g.pv.g, o.metric_tensor_pv)):
+        # the outermost ghost edge of N=8 sits at x = pi/2, where g -> 0: compare on the array scale
+        assert np.abs(R(_b6(mine)) - ref).max() <= 3e-15 * np.abs(ref).max()
+    for P, sfx in ((g.pc, "pc"), (g.pu, "pu"), (g.pv, "pv")):
+        for mine, name in ((P.ex_lon, "prod_ex_elon_"), (P.ex_lat, "prod_ex_elat_"), (P.ey_lon, "prod_ey_elon_"),
+                           (P.ey_lat, "prod_ey_elat_"), (P.det, "determinant_ll2contra_")):
+            assert np.allclose(R(mine), getattr(o, name + sfx), rtol=0, atol=3e-15)
+    I = slice(o.i0, o.iend)
+    assert abs(g.a2 - np.sum(o.metric_tensor_pc[I, I, :] ** 2)) < 1e-12 * g.a2
+
+
+@pytest.mark.parametrize("N", [8, 16, 48])
+def test_lagrange_tables_bitwise_equal_reference(N):
+    w, k = lagrange_tables(Grid(N))
+    assert np.array_equal(w, GOLD["N%d_w_east" % N])
+    assert np.array_equal(k, GOLD["N%d_kmin_east" % N])
+
+
+@pytest.mark.parametrize("N", [24, 96, 384])
+def test_lagrange_tables_equal_oracle(N):
+    w, k = lagrange_tables(Grid(N))
+    wo, (ko, _) = olagrange.lagrange_poly_ghostcell_pc(ogrid.CubedSphere(N))
+    assert np.array_equal(w, wo[0]) and np.array_equal(k, ko[0])
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_orientation_reproduces_reference_halo(N):
+    orient = orientation(N)
+    M = N + 8
+    Q = GOLD["N%d_Q" % N]
+    for side, key in ((0, "east"), (1, "west"), (2, "north"), (3, "south")):
+        H = GOLD["N%d_halo_%s" % (N, key)]
+        for p in range(6):
+            for d in range(4):
+                g = d if side in (0, 2) else 3 - d   # reference strips store W/S farthest-first
+                for r in range(M):
+                    q, i, j = halo_source(orient, p, side, d, r)
+                    ref = H[g, r, p] if side in (0, 1) else H[r, g, p]
+                    assert Q[i, j, q] == ref
+
+
+@pytest.mark.parametrize("N", [8, 17, 384, 1024])
+def test_orientation_and_stencils_stay_in_bounds(N):
+    """Every halo source index lies inside [0, M) for d in 0..3 and r in 0..M-1."""
+    o, M = orientation(N), N + 8
+    d = np.arange(4)[:, None]
+    r = np.arange(M)[None, :]
+    for p in range(6):
+        for s in range(4):
+            q, ib, id_, ir, jb, jd, jr = o[p, s]
+            i, j = ib + id_ * d + ir * r, jb + jd * d + jr * r
+            assert 0 <= q < 6 and q != p
+            assert i.min() >= 0 and i.max() < M and j.min() >= 0 and j.max() < M
+    if N <= 384:
+        _, k = lagrange_tables(Grid(N))
+        assert k.min() >= 0 and (k + 3).max() < M
+        assert k[:, 4:N + 4].min() >= 4 and (k[:, 4:N + 4] + 3).max() < N + 4
+
+
+@pytest.mark.parametrize("vf", [1, 2, 3, 4])
+def test_wind_basis_recombines_to_reference_wind(vf):
+    """sum_m s_m(t) b_m(lon, lat) equals the reference wind (src/advection_ic.py:287-313)."""
+    rng = np.random.default_rng(vf)
+    lon, lat = rng.uniform(-math.pi, math.pi, 500), rng.uniform(-1.5, 1.5, 500)
+    case = reference_case(2, vf)
+    for t in (0.0, 0.37, 2.9):
+        s = case.coefficients(t)
+        u = sum(sm * fu(lon, lat) for sm, (fu, _) in zip(s, case.basis()))
+        v = sum(sm * fv(lon, lat) for sm, (_, fv) in zip(s, case.basis()))
+        ur, vr = ocases.velocity_adv(lon, lat, t, vf)
+        assert np.allclose(u, ur, atol=1e-13) and np.allclose(v, vr, atol=1e-13)
+
+
+def test_extension_winds_and_fields_match_oracle():
+    rng = np.random.default_rng(11)
+    lon, lat = rng.uniform(-math.pi, math.pi, 400), rng.uniform(-1.5, 1.5, 400)
+    for name in ("solid_body_polar", "two_hills_tilted", "nl10_deformation"):
+        case = workload(name)
+        ocase = ocases.from_case(case)
+        for t in (0.0, 1.1):
+            s = case.coefficients(t)
+            u = sum(sm * fu(lon, lat) for sm, (fu, _) in zip(s, case.basis()))
+            ur, _ = ocases.case_wind(ocase, lon, lat, t)
+            assert np.allclose(u, ur, atol=1e-13), name
+            for n in range(case.ntr):
+                a = case.q(n, lon, lat, t)
+                b = ocases.case_q(ocase, ocases.case_tracers(ocase)[n], lon, lat, t)
+                assert np.allclose(a, b, atol=1e-13), (name, n)
+
+
+def test_coefficient_table_rows_follow_update_order():
+    case = reference_case(2, 3)
+    tab = case.coefficient_table(4, 0.1)
+    assert tab.shape == (4, 2, 1)
+    assert tab[0, 0, 0] == tab[0, 1, 0] == 1.0                       # step 1: winds at t=0, old = new
+    assert abs(tab[2, 0, 0] - math.cos(math.pi * 0.2 / 5.0)) < 1e-15  # step 3: t_2
+    assert abs(tab[2, 1, 0] - math.cos(math.pi * 0.1 / 5.0)) < 1e-15  # old: t_1
+
+
+def test_random_batched_case_is_seeded():
+    a = random_solid_body(np.random.default_rng(5), 3)
+    b = random_solid_body(np.random.default_rng(5), 3)
+    assert a.wind == b.wind and a.tracers == b.tracers
+    assert -math.pi / 2 <= a.wind[1] <= 0.0 and all(5.0 <= t["b"] <= 20.0 for t in a.tracers)
+
+
+# ------------------------------------------------------------------ par files
+def test_par_files_positional(tmp_path):
+    (tmp_path / "configuration.par").write_text("a\nb\n48\nc\n2\nd\n0\ne\n1\nf\n5\ng\n2\n")
+    (tmp_path / "advection.par").write_text(
+        "\n".join(["x", "y", "5", "#", "0.01", "#", "3", "#", "2", "#", "1", "#", "4", "#", "2", "#", "2", "#",
+                   "3", "#", "1", "#", "3"]) + "\n")
+    gp = config.get_parameters(str(tmp_path))
+    assert gp.N == 48 and gp.transformation == "gnomonic_equiangular" and gp.test_case == 5
+    ap = config.get_advection_parameters(str(tmp_path))
+    assert (ap.Tf, ap.dt, ap.ic, ap.vf, ap.recon, ap.dp, ap.opsplit, ap.et, ap.mt, ap.mf) == \
+        (5.0, 0.01, 3, 2, 4, 2, 2, 3, 1, 3)
+    config.check_options(ap)
+
+
+def test_repo_par_files_are_reference_defaults():
+    ap = config.get_advection_parameters(os.path.join(ROOT, "par"))
+    assert (ap.Tf, ap.dt, ap.ic, ap.vf, ap.tc, ap.recon, ap.dp, ap.opsplit, ap.et, ap.mt, ap.mf) == \
+        (5.0, 0.0025, 2, 1, 2, 3, 1, 1, 3, 1, 3)
+
+
+@pytest.mark.parametrize("bad", [dict(opsplit=3, mt=1), dict(opsplit=1, mt=2), dict(recon=5), dict(et=0)])
+def test_invalid_options_raise(bad):
+    p = config.AdvectionParams(0.01, 5.0, 1, 2, 1, 3, 1, 1, 3, 1, 3)
+    for k, v in bad.items():
+        setattr(p, k, v)
+    with pytest.raises(ValueError):
+        config.check_options(p)
+
+
+def test_missing_par_file_raises(tmp_path):
+    with pytest.raises(FileNotFoundError):
+        config.get_advection_parameters(str(tmp_path))
+
+
+# ------------------------------------------------------------------ C ABI
+def test_library_loads_and_exports_every_declared_symbol():
+    from cubedsphere import _lib
+    header = open(os.path.join(ROOT, "include", "csadv.h")).read()
+    declared = sorted(set(re.findall(r"\b(csadv_[a-z_]+)\s*\(", header)))
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_lib.SYMBOLS)
+    loaded = _lib.load()
+    assert loaded.csadv_abi_version() == 1
+    assert loaded.csadv_partials_per_tracer(384, 0) > 0
+    assert loaded.csadv_partials_per_tracer(384, 99) < 0
+
+
+def test_descriptor_struct_matches_header_field_order():
+    from cubedsphere import _lib
+    header = open(os.path.join(ROOT, "include", "csadv.h")).read()
+    body = header[header.index("typedef struct csadv_solve_desc"):header.index("} csadv_solve_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"\*?\s*([a-zA-Z_0-9]+)\s*(?=[,;])", body.split("{", 1)[1])
+    assert names == [f[0] for f in _lib.SolveDesc._fields_]
+
+
+def test_create_rejects_invalid_descriptor_without_gpu_work():
+    from cubedsphere import _lib
+    lib = _lib.load()
+    d = _lib.SolveDesc()
+    d.N, d.ntr, d.ld, d.ldu = 16, 1, 24, 28
+    d.recon, d.dp, d.split, d.et, d.mt, d.mf, d.nbasis = 3, 1, 3, 3, 1, 3, 1   # split 3 needs mt 2
+    h = ctypes.c_void_p()
+    rc = lib.csadv_create(ctypes.byref(d), 1, 0, ctypes.byref(h))
+    assert rc == -1
+    assert b"opsplit" in lib.csadv_last_error(h)
+    lib.csadv_destroy(h)
