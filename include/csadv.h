@@ -70,6 +70,15 @@ typedef struct csadv_solve_desc {
   const double *conv_pu;           /* [5][6][M][ldu]  same four and the determinant at x-edges     */
   const double *conv_pv;           /* [5][6][M+1][ld] same at y-edges                               */
   double *wpc;                     /* scratch [nbasis][2][6][M][ld]: lat-lon winds at centres      */
+  int32_t n_strip, n_corner;       /* ghost recipes: edge-strip and corner cells          */
+  const int32_t *strip_tgt;        /* [n_strip] target offsets inside one tracer           */
+  const int32_t *strip_src;        /* [n_strip][4] source offsets inside one tracer        */
+  const double *strip_w;           /* [n_strip][4] weights                                  */
+  const double *strip_gw;          /* [n_strip] sum_l w_l g(src_l): mass-fixer correction   */
+  const int32_t *corner_tgt;       /* [n_corner]                                            */
+  const int32_t *corner_src;       /* [n_corner][16]                                        */
+  const double *corner_w;          /* [n_corner][16]                                        */
+  const double *corner_gw;         /* [n_corner]                                            */
   const double *coef;              /* [max_steps][2][nbasis] basis coefficients (t_{n-1}, t_{n-2}) */
   double *corr;                    /* [ntr] pending mass-fixer correction (FP64)        */
   double *partials;                /* scratch, >= ntr * csadv_partials_per_tracer()     */

@@ -30,7 +30,7 @@ __global__ void k_ghost_edges(const FieldSet *sets) {
   const FieldSet &F = sets[blockIdx.y];
   const int p = blockIdx.x >> 4, side = (blockIdx.x >> 2) & 3, d = blockIdx.x & 3;
   const int N = F.N, M = F.M, ld = F.ld;
-  const int c = F.cur ? *F.cur : 0;
+  const int c = F.cur ? (*F.cur & 1) : 0;
   T *base = reinterpret_cast<T *>(F.base[c]);
   const int32_t *o = F.orient + (p * 4 + side) * 7;
   for (int kk = threadIdx.x; kk < N; kk += blockDim.x) {
@@ -71,7 +71,7 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
   const int side = t >> 5, d = (t >> 3) & 3, kk = t & 7;
   const int N = F.N, M = F.M, ld = F.ld;
   const int k = kk < 4 ? kk : N + NG + (kk - 4);
-  const int c = F.cur ? *F.cur : 0;
+  const int c = F.cur ? (*F.cur & 1) : 0;
   T *base = reinterpret_cast<T *>(F.base[c]);
   const int32_t *o = F.orient + (p * 4 + side) * 7;
   const int K = F.lagk[d * M + k];
@@ -99,30 +99,115 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
   }
 }
 
-// ------------------------------------------------------------------ fused step
-// Work item = (solve, panel, tile_x, tile_y).  One block computes TX x TY interior cells of
-// every tracer: region of Q with a 3-cell halo in LDS, x-pass and y-pass fluxes of Q, the
-// split half-steps Qx / Qy, the swapped second pass, the update and the MF-PR partial sum.
-template <typename T, int RECON, int TX, int TY>
-__global__ __launch_bounds__(BLOCK) void k_step(const Solve *solves, const int4 *work) {
-  constexpr int RW = TX + 6, RH = TY + 6;          // region with the 3-cell halo
-  constexpr int FW = (TX + 6 > TX + 1 ? TX + 6 : TX + 1);
-  __shared__ T sQ[RH][RW + 1];
-  __shared__ T sA[RH][TX + 1];                      // Qx rows tj-3 .. tj+TY+2, interior columns
-  __shared__ T sB[TY][RW + 1];                      // Qy interior rows, columns ti-3 .. ti+TX+2
-  __shared__ T sF[(RH > TY + 1 ? RH : TY + 1) * (FW + 1)];
-  __shared__ double sred[BLOCK / 64];
+// ------------------------------------------------------------------ step ghost fill
+// Ghost cells a step reads (3 layers) from flattened duo-grid recipes: every value is a
+// weighted sum of interior cells of other panels (edge strips: 4 sources; corners: 16, the
+// two interpolation levels of src/interpolation.py:250-314 multiplied out).  Interior
+// sources carry the pending mass-fixer correction: sum_l w_l (Q_l + c g_l) = sum_l w_l Q_l + c gw.
+// grid: x = recipe cells (strips then corners), y = solve.
+// Control words of a solve (device int32 ctl[4], zero at init), each written by one kernel
+// only and read by the other, so no kernel reads a word it writes:
+//   ctl[1]  steps completed           written by k_step (one lane), read by k_ghost, k_wind_avg
+//   ctl[2]  index of the running step written by k_ghost (one lane), read by k_step
+//   ctl[3]  1 = the partials of the last step are not yet folded into corr
+// The state of step n lives in q[n & 1].
+__device__ __forceinline__ double block_sum_fixed(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double s = red[0];
+  __syncthreads();
+  return s;
+}
 
+// Mass-fixer correction of the last step: c = dt * m0 / a2 with m0 = sum of div g over the
+// interior (src/discrete_operators.py:98-101), summed in a fixed order so that every block
+// (and every run) gets the same bits.
+__device__ double pending_corr(const Solve &S, int tr, double *red) {
+  const csadv_solve_desc &D = S.d;
+  if (D.mf != 3) return 0.0;
+  if (!D.ctl[3]) return D.corr[tr];
+  const double *P = D.partials + (int64_t)tr * S.part_per_tracer;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < S.part_per_tracer; i += BLOCK) acc += P[i];
+  return D.dt * block_sum_fixed(acc, red) / D.a2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
+  __shared__ double red[BLOCK];
+  const Solve &S = solves[blockIdx.y];
+  const csadv_solve_desc &D = S.d;
+  const int ncell = D.n_strip + D.n_corner;
+  if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
+  const int steps = D.ctl[1];
+  T *Q = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0);
+  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = idx < ncell;
+  const bool strip = idx < D.n_strip;
+  const int c = strip ? idx : idx - D.n_strip;
+  const int n = strip ? 4 : 16;
+  const int32_t *src = strip ? D.strip_src + 4 * c : D.corner_src + 16 * c;
+  const double *w = strip ? D.strip_w + 4 * c : D.corner_w + 16 * c;
+  for (int tr = 0; tr < D.ntr; ++tr) {
+    const double cf = pending_corr(S, tr, red);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && D.ctl[3]) D.corr[tr] = cf;
+    if (!active) continue;
+    T *Qt = Q + tr * S.tstride;
+    double acc = 0.0;
+    for (int l = 0; l < n; ++l) acc += (double)Qt[src[l]] * w[l];
+    acc += cf * (strip ? D.strip_gw[c] : D.corner_gw[c]);
+    Qt[strip ? D.strip_tgt[c] : D.corner_tgt[c]] = (T)acc;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
+}
+
+// Fold the last step's partials into corr (once; used by finalize and the diagnostics).
+__global__ void k_settle(Solve *solves) {
+  __shared__ double red[BLOCK];
+  Solve &S = solves[blockIdx.x];
+  csadv_solve_desc &D = S.d;
+  if (D.mf != 3 || !D.ctl[3]) return;
+  for (int tr = 0; tr < D.ntr; ++tr) {
+    const double cf = pending_corr(S, tr, red);
+    if (threadIdx.x == 0) D.corr[tr] = cf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) D.ctl[3] = 0;
+}
+
+// Work item = (solve, panel, strip, segment): one wavefront marches down strip_w interior
+// columns (lane l <-> column ti-3+l, so the 3-cell halo of each side rides in the outer
+// lanes) over seg_l interior rows.  Iteration y:
+//   * column stage at y-edge e = y-3 from 6-row register windows (rows y-6 .. y-1): y-fluxes
+//     of Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by lane shuffles), update of
+//     row y-4 and its mass-fixer partial sum;
+//   * row stage for row y: load (or ghost fill), x-flux of Q, split half-step Qx.
+// The two stages are independent within an iteration.  The last wavefront to finish sums
+// the partials in a fixed order, sets the pending correction, flips the state buffer and
+// advances the step counter (src/discrete_operators.py:18-101, src/advection_timestep.py:43).
+template <typename T, int RECON>
+__global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *work) {
+  const int lane = threadIdx.x;
   const int4 wk = work[blockIdx.x];
   const Solve &S = solves[wk.x];
   const csadv_solve_desc &D = S.d;
   const int p = wk.y;
-  const int N = D.N, ld = D.ld, ldu = D.ldu;
-  const int ti = NG + wk.z * TX, tj = NG + wk.w * TY;
-  const int nx = min(TX, N + NG - ti), ny = min(TY, N + NG - tj);
-  const int cur = D.ctl[0];
-  const T *Qin = reinterpret_cast<const T *>(cur ? D.q1 : D.q0);
-  T *Qout = reinterpret_cast<T *>(cur ? D.q0 : D.q1);
+  const int N = D.N, M = S.M, ld = D.ld, ldu = D.ldu;
+  const int iend = N + NG;
+  const int ti = NG + wk.z * S.strip_w, tj = NG + wk.w * S.seg_l;
+  const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
+  const int x = ti - 3 + lane;
+  const int xc = min(x, M - 1), xe = min(x, M);          // clamped (lanes past the array)
+  const bool out_lane = lane >= 3 && lane < 3 + nx;
+  const bool in_col = x >= NG && x < iend;
+  const bool need_col = x >= 1 && x < iend + 3;          // columns the stencils reach
+  const int step = D.ctl[2];
+  const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
+  T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
   const T *U = reinterpret_cast<const T *>(D.uavg) + p * S.pstride_u;
   const T *V = reinterpret_cast<const T *>(D.vavg) + p * S.pstride_v;
   const T *US = D.usgn ? reinterpret_cast<const T *>(D.usgn) + p * S.pstride_u : U;
@@ -132,206 +217,100 @@ __global__ __launch_bounds__(BLOCK) void k_step(const Solve *solves, const int4 
   const T *gv = reinterpret_cast<const T *>(D.gv);
   const T *rgc = reinterpret_cast<const T *>(D.rgc);
   const T dt = T(D.dt), rdx = T(1.0 / D.dx), rdy = T(1.0 / D.dy);
-  const T dtdx = T(D.dt / D.dx), dtdy = T(D.dt / D.dy);
   const bool mt0 = D.mt == 1;
   const int split = D.split;
   const bool mfpr = D.mf == 3;
-  const int tid = threadIdx.x;
-  const int x0 = ti - 3, y0 = tj - 3;               // region origin (global indices)
-  const int M = S.M;
-  const int tile_id = (p * S.tiles_y + wk.w) * S.tiles_x + wk.z;
+  const int item = (p * S.segs + wk.w) * S.strips + wk.z;
+  if (item == 0 && lane == 0) {      // hand the step over to the next kernels of the stream
+    D.ctl[1] = step + 1;
+    if (mfpr) D.ctl[3] = 1;
+  }
+  const int lm3 = max(lane - 3, 0), lm2 = max(lane - 2, 0), lm1 = max(lane - 1, 0);
+  const int lp1 = min(lane + 1, WAVE - 1), lp2 = min(lane + 2, WAVE - 1);
 
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const T *Q = Qin + tr * S.tstride + p * S.pstride;
-    const double cf = D.corr ? D.corr[tr] : 0.0;
-
-    // 1. region load; interior cells carry the pending mass-fixer correction
-    for (int idx = tid; idx < RH * RW; idx += BLOCK) {
-      const int r = idx / RW, cidx = idx - r * RW;
-      const int gj = y0 + r, gi = x0 + cidx;
-      T val = T(0);
-      if (gi < M && gj < M) {
-        val = Q[(int64_t)gj * ld + gi];
-        if (cf != 0.0 && gi >= NG && gi < N + NG && gj >= NG && gj < N + NG)
-          val = (T)((double)val + cf * D.gc64[(int64_t)gj * ld + gi]);
-      }
-      sQ[r][cidx] = val;
-    }
-    __syncthreads();
-
-    // 2. x-pass 1: fluxes of Q at edges ti..ti+nx of every region row
-    for (int idx = tid; idx < RH * (TX + 1); idx += BLOCK) {
-      const int r = idx / (TX + 1), e = idx - r * (TX + 1);
-      const int gj = y0 + r, ge = ti + e;
-      T f = T(0);
-      if (e <= nx && gj < M) {
-        const T u = U[(int64_t)gj * ldu + ge];
-        const bool up = US[(int64_t)gj * ldu + ge] >= T(0);
-        const int cgl = up ? ge - 1 : ge;               // upwind cell (global)
-        f = upwind_flux<T, RECON>(&sQ[r][cgl - x0], 1, up, (u * dt) * rdx, u,
-                                  gu[(int64_t)gj * ldu + cgl], gu[(int64_t)gj * ldu + cgl + 1],
-                                  gc[(int64_t)gj * ld + cgl], mt0, gu[(int64_t)gj * ldu + ge]);
-      }
-      sF[r * (FW + 1) + e] = f;
-    }
-    __syncthreads();
-
-    // 3. Qx = split half-step in x (interior columns, every region row)
-    for (int idx = tid; idx < RH * TX; idx += BLOCK) {
-      const int r = idx / TX, c = idx - r * TX;
-      const int gj = y0 + r, gi = ti + c;
-      T v = T(0);
-      if (c < nx && gj < M) {
-        const T q = sQ[r][c + 3];
-        const T dF = (-(sF[r * (FW + 1) + c + 1] - sF[r * (FW + 1) + c]) * dt) * rdx;
-        T c1 = T(0), c2 = T(0);
-        if (split != 1) {
-          c1 = gu[(int64_t)gj * ldu + gi + 1] * ((U[(int64_t)gj * ldu + gi + 1] * dt) * rdx);
-          c2 = gu[(int64_t)gj * ldu + gi] * ((U[(int64_t)gj * ldu + gi] * dt) * rdx);
-        }
-        v = split_update<T>(split, q, dF, gc[(int64_t)gj * ld + gi], rgc[(int64_t)gj * ld + gi], c1, c2);
-      }
-      sA[r][c] = v;
-    }
-    __syncthreads();
-
-    // 4. y-pass 1: fluxes of Q at y-edges tj..tj+ny of every region column
-    for (int idx = tid; idx < (TY + 1) * RW; idx += BLOCK) {
-      const int e = idx / RW, c = idx - e * RW;
-      const int ge = tj + e, gi = x0 + c;
-      T f = T(0);
-      if (e <= ny && gi < M) {
-        const T v = V[(int64_t)ge * ld + gi];
-        const bool up = VS[(int64_t)ge * ld + gi] >= T(0);
-        const int cgl = up ? ge - 1 : ge;
-        f = upwind_flux<T, RECON>(&sQ[cgl - y0][c], RW + 1, up, (v * dt) * rdy, v,
-                                  gv[(int64_t)cgl * ld + gi], gv[(int64_t)(cgl + 1) * ld + gi],
-                                  gc[(int64_t)cgl * ld + gi], mt0, gv[(int64_t)ge * ld + gi]);
-      }
-      sF[e * (FW + 1) + c] = f;
-    }
-    __syncthreads();
-
-    // 5. Qy = split half-step in y (interior rows, every region column)
-    for (int idx = tid; idx < TY * RW; idx += BLOCK) {
-      const int r = idx / RW, c = idx - r * RW;
-      const int gj = tj + r, gi = x0 + c;
-      T v = T(0);
-      if (r < ny && gi < M) {
-        const T q = sQ[r + 3][c];
-        const T dG = (-(sF[(r + 1) * (FW + 1) + c] - sF[r * (FW + 1) + c]) * dt) * rdy;
-        T c1 = T(0), c2 = T(0);
-        if (split != 1) {
-          c1 = gv[(int64_t)(gj + 1) * ld + gi] * ((V[(int64_t)(gj + 1) * ld + gi] * dt) * rdy);
-          c2 = gv[(int64_t)gj * ld + gi] * ((V[(int64_t)gj * ld + gi] * dt) * rdy);
-        }
-        v = split_update<T>(split, q, dG, gc[(int64_t)gj * ld + gi], rgc[(int64_t)gj * ld + gi], c1, c2);
-      }
-      sB[r][c] = v;
-    }
-    __syncthreads();
-
-    // 6. x-pass 2: fluxes of Qy at edges ti..ti+nx of the interior rows
-    for (int idx = tid; idx < TY * (TX + 1); idx += BLOCK) {
-      const int r = idx / (TX + 1), e = idx - r * (TX + 1);
-      const int gj = tj + r, ge = ti + e;
-      T f = T(0);
-      if (r < ny && e <= nx) {
-        const T u = U[(int64_t)gj * ldu + ge];
-        const bool up = US[(int64_t)gj * ldu + ge] >= T(0);
-        const int cgl = up ? ge - 1 : ge;
-        f = upwind_flux<T, RECON>(&sB[r][cgl - x0], 1, up, (u * dt) * rdx, u,
-                                  gu[(int64_t)gj * ldu + cgl], gu[(int64_t)gj * ldu + cgl + 1],
-                                  gc[(int64_t)gj * ld + cgl], mt0, gu[(int64_t)gj * ldu + ge]);
-      }
-      sF[r * (FW + 1) + e] = f;
-    }
-    __syncthreads();
-
-    // 7. keep dF2 of the owned cells in registers
-    constexpr int PER = (TX * TY + BLOCK - 1) / BLOCK;
-    T dF2[PER];
-#pragma unroll
-    for (int s = 0; s < PER; ++s) {
-      const int idx = tid + s * BLOCK;
-      const int r = idx / TX, c = idx - r * TX;
-      dF2[s] = (idx < TX * TY) ? (-(sF[r * (FW + 1) + c + 1] - sF[r * (FW + 1) + c]) * dt) * rdx : T(0);
-    }
-    __syncthreads();
-
-    // 8. y-pass 2: fluxes of Qx at y-edges tj..tj+ny of the interior columns
-    for (int idx = tid; idx < (TY + 1) * TX; idx += BLOCK) {
-      const int e = idx / TX, c = idx - e * TX;
-      const int ge = tj + e, gi = ti + c;
-      T f = T(0);
-      if (e <= ny && c < nx) {
-        const T v = V[(int64_t)ge * ld + gi];
-        const bool up = VS[(int64_t)ge * ld + gi] >= T(0);
-        const int cgl = up ? ge - 1 : ge;
-        f = upwind_flux<T, RECON>(&sA[cgl - y0][c], TX + 1, up, (v * dt) * rdy, v,
-                                  gv[(int64_t)cgl * ld + gi], gv[(int64_t)(cgl + 1) * ld + gi],
-                                  gc[(int64_t)cgl * ld + gi], mt0, gv[(int64_t)ge * ld + gi]);
-      }
-      sF[e * (FW + 1) + c] = f;
-    }
-    __syncthreads();
-
-    // 9. divergence, update, mass-fixer partial sum (src/discrete_operators.py:95-101,
-    //    src/advection_timestep.py:43)
-    double part = 0.0;
+    const T *Qt = Qin + tr * S.tstride;
+      const T *Q = Qt + p * S.pstride;
     T *Qo = Qout + tr * S.tstride + p * S.pstride;
+    const double cf = D.corr ? D.corr[tr] : 0.0;
+    T qw[6], xw[6];                 // Q and Qx of rows y-6 .. y-1 of this column
 #pragma unroll
-    for (int s = 0; s < PER; ++s) {
-      const int idx = tid + s * BLOCK;
-      const int r = idx / TX, c = idx - r * TX;
-      if (idx < TX * TY && r < ny && c < nx) {
-        const int gj = tj + r, gi = ti + c;
-        const T dG2 = (-(sF[(r + 1) * (FW + 1) + c] - sF[r * (FW + 1) + c]) * dt) * rdy;
-        const T g = gc[(int64_t)gj * ld + gi];
-        const T div = -(dF2[s] + dG2) / (dt * g);
-        Qo[(int64_t)gj * ld + gi] = sQ[r + 3][c + 3] - dt * div;
-        if (mfpr) part += (double)div * (double)g;
+    for (int k = 0; k < 6; ++k) { qw[k] = T(0); xw[k] = T(0); }
+    T g1prev = T(0), g2prev = T(0), vprev = T(0);
+    double part = 0.0;
+
+    for (int y = tj - 3; y <= tj + ny + 3; ++y) {
+      // ---- column stage: y-edge e = y-3, output row r = y-4
+      if (y >= tj + 3) {
+        const int e = y - 3;
+        const T v = V[(int64_t)e * ld + xc];
+        const bool vup = VS[(int64_t)e * ld + xc] >= T(0);
+        const T gvm1 = gv[(int64_t)(e - 1) * ld + xc], gv0 = gv[(int64_t)e * ld + xc], gvp1 = gv[(int64_t)(e + 1) * ld + xc];
+        const T gcm = gc[(int64_t)(e - 1) * ld + xc], gce = gc[(int64_t)e * ld + xc];
+        const T cy = (v * dt) * rdy;
+        const T g1 = edge_flux<T, RECON>(qw[0], qw[1], qw[2], qw[3], qw[4], qw[5], vup, cy, v, gvm1, gv0, gvp1, gcm, gce, mt0);
+        const T g2 = edge_flux<T, RECON>(xw[0], xw[1], xw[2], xw[3], xw[4], xw[5], vup, cy, v, gvm1, gv0, gvp1, gcm, gce, mt0);
+        if (y >= tj + 4) {
+          const int r = e - 1;
+          const T dG1 = (-(g1 - g1prev) * dt) * rdy;
+          T d1 = T(0), d2 = T(0);
+          if (split != 1) {
+            d1 = gv0 * ((v * dt) * rdy);
+            d2 = gvm1 * ((vprev * dt) * rdy);
+          }
+          const T rg = rgc[(int64_t)r * ld + xc];
+          const T qy = split_update<T>(split, qw[2], dG1, gcm, rg, d1, d2);
+          const T ur = U[(int64_t)r * ldu + xe];
+          const bool urp = US[(int64_t)r * ldu + xe] >= T(0);
+          const T gur = gu[(int64_t)r * ldu + xe];
+          const T f2 = edge_flux<T, RECON>(lane_shfl(qy, lm3), lane_shfl(qy, lm2), lane_shfl(qy, lm1), qy,
+                                           lane_shfl(qy, lp1), lane_shfl(qy, lp2), urp, (ur * dt) * rdx, ur,
+                                           lane_shfl(gur, lm1), gur, lane_shfl(gur, lp1), lane_shfl(gcm, lm1), gcm, mt0);
+          const T dF2 = (-(lane_shfl(f2, lp1) - f2) * dt) * rdx;
+          const T dG2 = (-(g2 - g2prev) * dt) * rdy;
+          // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
+          const T flx = dF2 + dG2;
+          if (out_lane) {
+            Qo[(int64_t)r * ld + x] = qw[2] + flx * rg;
+            if (mfpr) part -= (double)flx;
+          }
+        }
+        g1prev = g1;
+        g2prev = g2;
+        vprev = v;
       }
+      // ---- row stage: row y (ghost cells were filled by k_ghost; interior cells carry the
+      //      pending mass-fixer correction)
+      T q = T(0);
+      if (need_col && y >= 1 && y < iend + 3) {
+        q = Q[(int64_t)y * ld + x];
+        if (cf != 0.0 && in_col && y >= NG && y < iend) q = (T)((double)q + cf * D.gc64[(int64_t)y * ld + x]);
+      }
+      const T u = U[(int64_t)y * ldu + xe];
+      const bool up = US[(int64_t)y * ldu + xe] >= T(0);
+      const T gu0 = gu[(int64_t)y * ldu + xe];
+      const T gc0 = gc[(int64_t)y * ld + xc];
+      const T gum1 = lane_shfl(gu0, lm1), gup1 = lane_shfl(gu0, lp1), gcm1 = lane_shfl(gc0, lm1);
+      const T f1 = edge_flux<T, RECON>(lane_shfl(q, lm3), lane_shfl(q, lm2), lane_shfl(q, lm1), q,
+                                       lane_shfl(q, lp1), lane_shfl(q, lp2), up, (u * dt) * rdx, u,
+                                       gum1, gu0, gup1, gcm1, gc0, mt0);
+      const T dF1 = (-(lane_shfl(f1, lp1) - f1) * dt) * rdx;
+      T c1 = T(0), c2 = T(0);
+      if (split != 1) {
+        const T un = lane_shfl(u, lp1);
+        c1 = gup1 * ((un * dt) * rdx);
+        c2 = gu0 * ((u * dt) * rdx);
+      }
+      const T qx = split_update<T>(split, q, dF1, gc0, rgc[(int64_t)y * ld + xc], c1, c2);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) { qw[k] = qw[k + 1]; xw[k] = xw[k + 1]; }
+      qw[5] = q;
+      xw[5] = qx;
     }
     if (mfpr) {
-      part = warp_sum(part);
-      if ((tid & 63) == 0) sred[tid >> 6] = part;
-      __syncthreads();
-      if (tid == 0) {
-        double s = 0.0;
-        for (int w = 0; w < BLOCK / 64; ++w) s += sred[w];
-        D.partials[(int64_t)tr * S.part_per_tracer + tile_id] = s;
-      }
+      part = warp_sum(part) / D.dt;           // sum of div g over the wavefront's cells
+      if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
     }
-    __syncthreads();
-  }
-}
-
-// One block per solve: fixed-order sums of the tile partials -> pending correction
-// c = dt * m0 / a2 (Q^{n+1} = Q* + c g, src/discrete_operators.py:98-101); then flip the
-// current buffer and advance the step counter.
-__global__ void k_step_reduce(Solve *solves) {
-  Solve &S = solves[blockIdx.x];
-  csadv_solve_desc &D = S.d;
-  __shared__ double red[BLOCK];
-  if (D.mf == 3) {
-    for (int tr = 0; tr < D.ntr; ++tr) {
-      double s = 0.0;
-      const double *P = D.partials + (int64_t)tr * S.part_per_tracer;
-      for (int i = threadIdx.x; i < S.part_per_tracer; i += BLOCK) s += P[i];
-      red[threadIdx.x] = s;
-      __syncthreads();
-      for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) D.corr[tr] = D.dt * red[0] / D.a2;
-      __syncthreads();
-    }
-  }
-  if (threadIdx.x == 0) {
-    D.ctl[0] ^= 1;
-    D.ctl[1] += 1;
   }
 }
 
@@ -342,7 +321,7 @@ __global__ void k_finalize(Solve *solves) {
   csadv_solve_desc &D = S.d;
   if (D.mf != 3) return;
   const int N = D.N, ld = D.ld;
-  T *Q = reinterpret_cast<T *>(D.ctl[0] ? D.q1 : D.q0);
+  T *Q = reinterpret_cast<T *>((D.ctl[1] & 1) ? D.q1 : D.q0);
   const int64_t tot = (int64_t)6 * N * N;
   for (int tr = 0; tr < D.ntr; ++tr) {
     const double cf = D.corr[tr];
@@ -475,7 +454,7 @@ __global__ void k_wind_avg(Solve *solves) {
   const int N = D.N, M = S.M, ld = D.ld, ldu = D.ldu, K = D.nbasis;
   const int p = blockIdx.y;
   const int i0 = NG, iend = N + NG;
-  const int row = D.unsteady ? min(D.ctl[1], D.max_steps - 1) : 0;
+  const int row = D.unsteady ? min(D.ctl[1], D.max_steps - 1) : 0;   // steps completed
   const double *s1 = D.coef + (int64_t)row * 2 * K;   // coefficients at t_{n-1}
   const double *s0 = s1 + K;                          // at t_{n-2}
   const double dto2 = 0.5 * D.dt;
@@ -533,7 +512,7 @@ __global__ void k_mass(const Solve *solves, int solve, int tracer, double *parti
   const Solve &S = solves[solve];
   const csadv_solve_desc &D = S.d;
   const int N = D.N, ld = D.ld;
-  const T *Q = reinterpret_cast<const T *>(D.ctl[0] ? D.q1 : D.q0) + tracer * S.tstride;
+  const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
   const double cf = D.corr ? D.corr[tracer] : 0.0;
   const double dxdy = D.dx * D.dy;
   KSum acc{0.0, 0.0};
@@ -563,7 +542,7 @@ __global__ void k_norms(const Solve *solves, int solve, int tracer, const double
   const Solve &S = solves[solve];
   const csadv_solve_desc &D = S.d;
   const int N = D.N, ld = D.ld;
-  const T *Q = reinterpret_cast<const T *>(D.ctl[0] ? D.q1 : D.q0) + tracer * S.tstride;
+  const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
   const double cf = D.corr ? D.corr[tracer] : 0.0;
   KSum s1{0.0, 0.0}, s2{0.0, 0.0};
   double mx = 0.0;
@@ -612,9 +591,16 @@ struct csadv_s {
 
 namespace {
 
-struct TileShape { int tx, ty; };
-constexpr TileShape kTiles[] = {{64, 16}, {32, 32}, {64, 8}, {128, 8}, {32, 16}};
-constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+// tile id -> rows per wavefront segment (the strip width is always the widest that fits)
+constexpr int kSegRows[] = {8, 4, 16, 32, 6, 12};
+constexpr int kNumTiles = sizeof(kSegRows) / sizeof(kSegRows[0]);
+
+void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs) {
+  strips = (N + STRIP_MAX - 1) / STRIP_MAX;
+  strip_w = (N + strips - 1) / strips;          // balanced strips of at most 58 columns
+  seg_l = std::min(kSegRows[tile], N);
+  segs = (N + seg_l - 1) / seg_l;
+}
 constexpr int kRedBlocks = 256;
 
 int fail(csadv_t *h, int code, const std::string &msg) {
@@ -651,6 +637,11 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     return CSADV_E_ARG;
   }
   if (d.mf == 3 && (!d.corr || !d.partials)) { msg = "MF-PR needs corr and partials"; return CSADV_E_ARG; }
+  if (d.n_strip != 6 * 4 * 3 * d.N || d.n_corner != 6 * 2 * 3 * 6 || !d.strip_tgt || !d.strip_src || !d.strip_w ||
+      !d.strip_gw || !d.corner_tgt || !d.corner_src || !d.corner_w || !d.corner_gw) {
+    msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";
+    return CSADV_E_ARG;
+  }
   if (!(d.dt > 0.0) || !(d.dx > 0.0) || !(d.dy > 0.0)) { msg = "dt, dx, dy must be positive"; return CSADV_E_ARG; }
   return CSADV_OK;
 }
@@ -659,33 +650,20 @@ Solve derive(const csadv_solve_desc &d, int tile) {
   Solve s{};
   s.d = d;
   s.M = d.N + 2 * NG;
-  const TileShape ts = kTiles[tile];
-  s.tiles_x = (d.N + ts.tx - 1) / ts.tx;
-  s.tiles_y = (d.N + ts.ty - 1) / ts.ty;
-  s.tiles = s.tiles_x * s.tiles_y;
+  tiling(d.N, tile, s.strip_w, s.strips, s.seg_l, s.segs);
+  s.items = s.strips * s.segs;
   s.pstride = (int64_t)s.M * d.ld;
   s.tstride = 6 * s.pstride;
   s.pstride_u = (int64_t)s.M * d.ldu;
   s.pstride_v = (int64_t)(s.M + 1) * d.ld;
-  s.part_per_tracer = 6 * s.tiles;
+  s.part_per_tracer = 6 * s.items;
   return s;
-}
-
-template <typename T, int R, int TX, int TY>
-hipError_t launch_step_t(csadv_t *h, hipStream_t st) {
-  hipLaunchKernelGGL((k_step<T, R, TX, TY>), dim3(h->nwork), dim3(BLOCK), 0, st, h->dev, h->work);
-  return hipGetLastError();
 }
 
 template <typename T, int R>
 hipError_t launch_step_r(csadv_t *h, hipStream_t st) {
-  switch (h->tile) {
-    case 0: return launch_step_t<T, R, 64, 16>(h, st);
-    case 1: return launch_step_t<T, R, 32, 32>(h, st);
-    case 2: return launch_step_t<T, R, 64, 8>(h, st);
-    case 3: return launch_step_t<T, R, 128, 8>(h, st);
-    default: return launch_step_t<T, R, 32, 16>(h, st);
-  }
+  hipLaunchKernelGGL((k_step<T, R>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  return hipGetLastError();
 }
 
 template <typename T>
@@ -696,6 +674,19 @@ hipError_t launch_step_dt(csadv_t *h, hipStream_t st) {
     case 4: return launch_step_r<T, 4>(h, st);
     default: return launch_step_r<T, 3>(h, st);
   }
+}
+
+int step_ghosts(csadv_t *h, hipStream_t st) {
+  const int ns = (int)h->desc.size();
+  int maxc = 0;
+  for (auto &d : h->desc) maxc = std::max(maxc, d.n_strip + d.n_corner);
+  const dim3 grid((maxc + BLOCK - 1) / BLOCK, ns);
+  if (h->dtype == CSADV_F64)
+    hipLaunchKernelGGL(k_ghost<double>, grid, dim3(BLOCK), 0, st, h->dev);
+  else
+    hipLaunchKernelGGL(k_ghost<float>, grid, dim3(BLOCK), 0, st, h->dev);
+  HIPCHK(h, hipGetLastError());
+  return CSADV_OK;
 }
 
 int ghost_fill(csadv_t *h, hipStream_t st) {
@@ -735,8 +726,9 @@ int csadv_abi_version(void) { return CSADV_ABI_VERSION; }
 
 int csadv_partials_per_tracer(int32_t N, int32_t tile) {
   if (tile < 0 || tile >= kNumTiles || N < 1) return CSADV_E_ARG;
-  const TileShape ts = kTiles[tile];
-  return 6 * ((N + ts.tx - 1) / ts.tx) * ((N + ts.ty - 1) / ts.ty);
+  int sw, ns, sl, nsg;
+  tiling(N, tile, sw, ns, sl, nsg);
+  return 6 * ns * nsg;
 }
 
 int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, csadv_t **out) {
@@ -789,12 +781,12 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
     Solve S = derive(solves[s], h->tile);
     h->host.push_back(S);
     for (int p = 0; p < 6; ++p)
-      for (int ty = 0; ty < S.tiles_y; ++ty)
-        for (int tx = 0; tx < S.tiles_x; ++tx) work.push_back(make_int4(s, p, tx, ty));
+      for (int sg = 0; sg < S.segs; ++sg)
+        for (int sx = 0; sx < S.strips; ++sx) work.push_back(make_int4(s, p, sx, sg));
     FieldSet F{};
     F.base[0] = solves[s].q0;
     F.base[1] = solves[s].q1;
-    F.cur = solves[s].ctl;
+    F.cur = solves[s].ctl + 1;
     F.nfield = solves[s].ntr;
     F.N = solves[s].N;
     F.M = S.M;
@@ -875,9 +867,9 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream) {
   hipStream_t st = (hipStream_t)stream;
   const int ns = (int)h->desc.size();
   for (int k = 0; k < nsteps; ++k) {
-    int rc = ghost_fill(h, st);
+    int rc = wind_avg(h, st, false);
     if (rc) return rc;
-    rc = wind_avg(h, st, false);
+    rc = step_ghosts(h, st);
     if (rc) return rc;
     const bool prof = 2 * (h->ev_used + 1) <= (int)h->ev.size();
     if (prof) HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used], st));
@@ -887,8 +879,6 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream) {
       HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used + 1], st));
       ++h->ev_used;
     }
-    hipLaunchKernelGGL(k_step_reduce, dim3(ns), dim3(BLOCK), 0, st, h->dev);
-    HIPCHK(h, hipGetLastError());
   }
   for (int s = 0; s < ns; ++s) h->cur[s] = (h->cur[s] + nsteps) & 1;
   return CSADV_OK;
@@ -898,6 +888,7 @@ int csadv_finalize(csadv_t *h, void *stream) {
   if (!h || !h->dev) return CSADV_E_STATE;
   hipStream_t st = (hipStream_t)stream;
   const int ns = (int)h->desc.size();
+  hipLaunchKernelGGL(k_settle, dim3(ns), dim3(BLOCK), 0, st, h->dev);
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_finalize<double>, dim3(256, ns), dim3(BLOCK), 0, st, h->dev);
   else
@@ -912,6 +903,7 @@ int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *str
   if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr || !out)
     return fail(h, CSADV_E_ARG, "mass: bad solve / tracer / output");
   hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_settle, dim3((int)h->desc.size()), dim3(BLOCK), 0, st, h->dev);
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_mass<double>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, h->red);
   else
@@ -931,6 +923,7 @@ int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact,
   if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr || !out || !qexact)
     return fail(h, CSADV_E_ARG, "norms: bad solve / tracer / pointer");
   hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_settle, dim3((int)h->desc.size()), dim3(BLOCK), 0, st, h->dev);
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_norms<double>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, qexact, h->red);
   else

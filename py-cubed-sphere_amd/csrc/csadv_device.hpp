@@ -15,12 +15,16 @@ constexpr int BLOCK = 256;     // threads per block of the step kernel (4 waves)
 struct Solve {
   csadv_solve_desc d;
   int32_t M;
-  int32_t tiles_x, tiles_y, tiles;   // step tiling of one panel
+  int32_t strip_w, seg_l;            // one wavefront: strip_w columns x seg_l rows of a panel
+  int32_t strips, segs, items;       // strips and segments per panel, items = strips * segs
   int64_t pstride, tstride;          // cell arrays: panel / tracer strides
   int64_t pstride_u, pstride_v;      // x-edge / y-edge arrays: panel strides
-  int32_t part_per_tracer;           // partial sums per tracer (= 6 * tiles)
+  int32_t part_per_tracer;           // partial sums per tracer (= 6 * items)
   int32_t pad;
 };
+
+constexpr int WAVE = 64;
+constexpr int STRIP_MAX = WAVE - 6;  // interior columns of one wavefront (3-cell halo each side)
 
 // A set of FP64 or dtype fields that receive a duo-grid ghost fill.
 struct FieldSet {
@@ -123,6 +127,36 @@ __device__ __forceinline__ T upwind_flux(const T *w, int s, bool upos, T cc, T u
   f = f * uav;
   if (!mt0) f = f * ge;
   return f;
+}
+
+// Same flux from the 6-cell neighbourhood of the edge between cells c-1 and c:
+// m3..p2 = Q[c-3..c+2]; the upwind cell (c-1 if up, else c) supplies the parabola.
+// ge_m1 / ge_0 / ge_p1: metric at edges c-1, c, c+1; gc_m1 / gc_0: metric at cells c-1, c.
+template <typename T, int RECON>
+__device__ __forceinline__ T edge_flux(T m3, T m2, T m1, T c0, T p1, T p2, bool up, T cc, T uav,
+                                       T ge_m1, T ge_0, T ge_p1, T gc_m1, T gc_0, bool mt0) {
+  const T w0 = up ? m3 : m2, w1 = up ? m2 : m1, w2 = up ? m1 : c0, w3 = up ? c0 : p1, w4 = up ? p1 : p2;
+  T qL, qR;
+  recon_edges<T, RECON>(w0, w1, w2, w3, w4, qL, qR);
+  T q = w2;
+  if (mt0) {
+    qL = qL * (up ? ge_m1 : ge_0);
+    qR = qR * (up ? ge_0 : ge_p1);
+    q = q * (up ? gc_m1 : gc_0);
+  }
+  const T dq = qR - qL;
+  const T q6 = T(6) * q - T(3) * (qR + qL);
+  const T third = T(1.0 / 3.0);
+  T f = up ? (qR + (cc * T(0.5)) * (q6 - dq) - ((q6 * cc) * cc) * third)
+           : (qL - (cc * T(0.5)) * (q6 + dq) - ((q6 * cc) * cc) * third);
+  f = f * uav;
+  if (!mt0) f = f * ge_0;
+  return f;
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_shfl(T v, int src) {
+  return __shfl(v, src, 64);
 }
 
 // Split update of one cell after the first directional pass (src/discrete_operators.py:49-73).

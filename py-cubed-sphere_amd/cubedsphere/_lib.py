@@ -37,6 +37,9 @@ class SolveDesc(ctypes.Structure):
         ("gc", _p), ("gu", _p), ("gv", _p), ("rgc", _p), ("gc64", _p),
         ("lagw", _p), ("lagk", _p), ("orient", _p), ("wbu", _p), ("wbv", _p),
         ("conv_pc", _p), ("conv_pu", _p), ("conv_pv", _p), ("wpc", _p),
+        ("n_strip", ctypes.c_int32), ("n_corner", ctypes.c_int32),
+        ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p), ("strip_gw", _p),
+        ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p), ("corner_gw", _p),
         ("coef", _p), ("corr", _p), ("partials", _p), ("ctl", _p),
     ]
 

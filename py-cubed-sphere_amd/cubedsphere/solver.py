@@ -15,7 +15,7 @@ import torch
 
 from . import _lib
 from .grid import Grid
-from .topology import lagrange_tables, orientation
+from .topology import ghost_recipes, lagrange_tables, orientation
 
 _TDT = {"f64": torch.float64, "f32": torch.float32}
 
@@ -61,6 +61,15 @@ class GridTables:
         self.lagw = up(w)
         self.lagk = up(k, torch.int32)
         self.orient = up(orientation(N), torch.int32)
+        rec = ghost_recipes(grid, ld)
+        gflat = _pad_last(g0, ld).reshape(-1)
+        self.n_strip, self.n_corner = len(rec["strip_tgt"]), len(rec["corner_tgt"])
+        self.strip_tgt, self.strip_src = up(rec["strip_tgt"], torch.int32), up(rec["strip_src"], torch.int32)
+        self.strip_w = up(rec["strip_w"])
+        self.strip_gw = up(np.sum(rec["strip_w"] * gflat[rec["strip_gsrc"]], axis=1))
+        self.corner_tgt, self.corner_src = up(rec["corner_tgt"], torch.int32), up(rec["corner_src"], torch.int32)
+        self.corner_w = up(rec["corner_w"])
+        self.corner_gw = up(np.sum(rec["corner_w"] * gflat[rec["corner_gsrc"]], axis=1))
         pc, pu, pv = grid.pc, grid.pu, grid.pv
         self.conv_pc = up(_pad_last(np.stack([pc.ex_lon, pc.ex_lat, pc.ey_lon, pc.ey_lat]), ld))
         self.conv_pu = up(_pad_last(np.stack([pu.ex_lon, pu.ex_lat, pu.ey_lon, pu.ey_lat, pu.det]), ldu))
@@ -167,6 +176,11 @@ class Advection:
         d.lagw, d.lagk, d.orient = _ptr(tab.lagw), _ptr(tab.lagk), _ptr(tab.orient)
         d.wbu, d.wbv = _ptr(self.wbu), _ptr(self.wbv)
         d.conv_pc, d.conv_pu, d.conv_pv, d.wpc = _ptr(tab.conv_pc), _ptr(tab.conv_pu), _ptr(tab.conv_pv), _ptr(self.wpc)
+        d.n_strip, d.n_corner = tab.n_strip, tab.n_corner
+        d.strip_tgt, d.strip_src, d.strip_w, d.strip_gw = (_ptr(tab.strip_tgt), _ptr(tab.strip_src), _ptr(tab.strip_w),
+                                                           _ptr(tab.strip_gw))
+        d.corner_tgt, d.corner_src, d.corner_w, d.corner_gw = (_ptr(tab.corner_tgt), _ptr(tab.corner_src),
+                                                               _ptr(tab.corner_w), _ptr(tab.corner_gw))
         d.coef, d.corr, d.partials, d.ctl = _ptr(self.coef), _ptr(self.corr), _ptr(self.partials), _ptr(self.ctl)
         return d
 

@@ -109,3 +109,94 @@ def lagrange_tables(grid, degree=3):
                 L = L * (xq[..., 0] - nodes[..., i]) / (nodes[..., j] - nodes[..., i])
         w[..., j] = L
     return np.ascontiguousarray(w), np.ascontiguousarray(kmin.astype(np.int32))
+
+
+def _ghost_target(side, d, k, N):
+    iend = N + 4
+    return {E: (iend + d, k), W: (3 - d, k), N_: (k, iend + d), S: (k, 3 - d)}[side]
+
+
+def ghost_recipes(grid, ld, depth=3):
+    """Flattened duo-grid recipes of the ghost cells a step reads (depth < 3).
+
+    Every ghost value is a weighted sum of interior cells of other panels:
+      strips : 4 sources (src/interpolation.py:191-248)
+      corners: up to 16 sources -- a corner interpolates along the east / west halo, whose
+               points may be the neighbour's strip ghosts, themselves 4-source sums
+               (src/interpolation.py:250-314); the two levels are multiplied out.
+    Returns dict of numpy arrays (offsets are element offsets inside one tracer:
+    panel * M * ld + j * ld + i):
+      strip_tgt [n] int32, strip_src [n][4] int32, strip_w [n][4] f64
+      corner_tgt [m] int32, corner_src [m][16] int32, corner_w [m][16] f64
+    ``*_gsrc`` hold the offsets inside one panel (j * ld + i) for the metric of each source.
+    Corner stencils may read a neighbour strip's 4th layer, so the strip lookup covers depth 3.
+    """
+    N, M = grid.N, grid.M
+    orient = orientation(N)
+    w, kmin = lagrange_tables(grid)
+    pstride = M * ld
+    ks = np.arange(4, N + 4)
+    # ---- strips
+    tgt, src, wts, gsrc = [], [], [], []
+    strip_of = {}
+    for p in range(6):
+        for side in (E, W, N_, S):
+            q, ib, id_, ir, jb, jd, jr = (int(v) for v in orient[p, side])
+            for d in range(4):   # corner stencils may reach the 4th layer of a strip
+                K = kmin[d, ks][:, None] + np.arange(4)[None, :]
+                si = ib + id_ * d + ir * K
+                sj = jb + jd * d + jr * K
+                ti, tj = _ghost_target(side, d, ks, N)
+                ti = np.broadcast_to(ti, ks.shape)
+                tj = np.broadcast_to(tj, ks.shape)
+                s_off = q * pstride + sj * ld + si
+                g_off = sj * ld + si
+                for n in range(len(ks)):
+                    strip_of[(p, int(ti[n]), int(tj[n]))] = (s_off[n], g_off[n], w[d, ks[n], :])
+                if d < depth:
+                    tgt.append(p * pstride + tj * ld + ti)
+                    src.append(s_off)
+                    gsrc.append(g_off)
+                    wts.append(w[d, ks, :])
+    out = {
+        "strip_tgt": np.concatenate(tgt).astype(np.int32),
+        "strip_src": np.concatenate(src).astype(np.int32),
+        "strip_gsrc": np.concatenate(gsrc).astype(np.int32),
+        "strip_w": np.ascontiguousarray(np.concatenate(wts)),
+    }
+    # ---- corners (east / west halos, rows outside the panel)
+    iend = N + 4
+    rows = [r for r in range(4 - depth, 4)] + [r for r in range(iend, iend + depth)]
+    ctgt, csrc, cgsrc, cw = [], [], [], []
+    for p in range(6):
+        for side in (E, W):
+            q, ib, id_, ir, jb, jd, jr = (int(v) for v in orient[p, side])
+            for d in range(depth):
+                for k in rows:
+                    ti, tj = _ghost_target(side, d, k, N)
+                    s_, g_, w_ = [], [], []
+                    for l in range(4):
+                        r = int(kmin[d, k]) + l
+                        si, sj = ib + id_ * d + ir * r, jb + jd * d + jr * r
+                        if 4 <= si < iend and 4 <= sj < iend:
+                            s_.append(q * pstride + sj * ld + si)
+                            g_.append(sj * ld + si)
+                            w_.append(w[d, k, l])
+                        else:
+                            ss, gg, ww = strip_of[(q, si, sj)]
+                            s_.extend(int(v) for v in ss)
+                            g_.extend(int(v) for v in gg)
+                            w_.extend(float(w[d, k, l] * v) for v in ww)
+                    while len(s_) < 16:          # pad with zero-weight copies of the first source
+                        s_.append(s_[0])
+                        g_.append(g_[0])
+                        w_.append(0.0)
+                    ctgt.append(p * pstride + tj * ld + ti)
+                    csrc.append(s_)
+                    cgsrc.append(g_)
+                    cw.append(w_)
+    out["corner_tgt"] = np.array(ctgt, dtype=np.int32)
+    out["corner_src"] = np.array(csrc, dtype=np.int32)
+    out["corner_gsrc"] = np.array(cgsrc, dtype=np.int32)
+    out["corner_w"] = np.array(cw, dtype=np.float64)
+    return out
