@@ -87,8 +87,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="two_hills_n384_f64", choices=sorted(WORKLOADS))
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=0,
+                    help="replay the timed steps from HIP graphs of this many steps (0 = plain launches); "
+                         "the step-kernel duration then comes from a separate plain-launch pass")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="copy a buffer of this many MiB device-to-device once before the timed steps "
+                         "(a kernel of known traffic for scaling hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -108,15 +114,31 @@ def main():
     case = workload(case_name)
     grid = Grid(N)
     dt = cfl_dt(grid, case, cfl)
-    sol = Advection(N, case, dt, dtype=dtype, device="cuda:0", tile=args.tile, max_steps=K + W, grid=grid)
+    sol = Advection(N, case, dt, dtype=dtype, device="cuda:0", tile=args.tile,
+                    max_steps=W + 2 * K + max(args.graph, 0), grid=grid)
     sol.init_vars_adv()
+    if args.calibrate_copy_mib > 0:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sol.step(W)
-    sol.profile(K)
+    if args.graph > 0:
+        sol.use_graph(args.graph)
+        sol.step(args.graph)                 # capture outside the timed region
+    else:
+        sol.profile(K)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sol.step(K)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if args.graph > 0:
+        sol.use_graph(0)
+        sol.profile(K)
+        sol.step(K)
     kern_ms, launches = sol.profile_read()
     sol.profile(0)
 

@@ -114,6 +114,14 @@ int csadv_ghost_fill(csadv_t *h, void *stream);
  * called nsteps times.  Graph-capturable (no host synchronisation, no allocation). */
 int csadv_step(csadv_t *h, int32_t nsteps, void *stream);
 
+/* Replay steps from a HIP graph: steps_per_graph > 0 makes csadv_step capture that many
+ * consecutive steps (all their kernels) once, on a private capture stream, and launch the
+ * instantiated graph on the caller's stream for every whole group of steps; remaining
+ * steps, and every step while csadv_profile pairs are armed, run as plain launches.
+ * Results are bitwise identical to plain launches.  0 (the default) switches graphs off
+ * and releases the graph.  No counterpart in the reference (its step is a Python call). */
+int csadv_set_graph(csadv_t *h, int32_t steps_per_graph);
+
 /* Fold the pending mass-fixer corrections into the current state buffer so that it
  * holds Q exactly as the reference would after the last step. */
 int csadv_finalize(csadv_t *h, void *stream);

@@ -131,8 +131,18 @@ __device__ double pending_corr(const Solve &S, int tr, double *red) {
   if (D.mf != 3) return 0.0;
   if (!D.ctl[3]) return D.corr[tr];
   const double *P = D.partials + (int64_t)tr * S.part_per_tracer;
+  const int n = S.part_per_tracer;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < S.part_per_tracer; i += BLOCK) acc += P[i];
+  int i = threadIdx.x;
+  // eight independent loads in flight per thread; the additions keep the order i, i+BLOCK, ...
+  for (; i + 7 * BLOCK < n; i += 8 * BLOCK) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = P[i + u * BLOCK];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; i < n; i += BLOCK) acc += P[i];
   return D.dt * block_sum_fixed(acc, red) / D.a2;
 }
 
@@ -158,7 +168,29 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
     if (!active) continue;
     T *Qt = Q + tr * S.tstride;
     double acc = 0.0;
-    for (int l = 0; l < n; ++l) acc += (double)Qt[src[l]] * w[l];
+    // all source indices, then all source values, are requested before the first use;
+    // the sum runs in recipe order either way
+    if (strip) {
+      const int4 s4 = *reinterpret_cast<const int4 *>(src);
+      const T v0 = Qt[s4.x], v1 = Qt[s4.y], v2 = Qt[s4.z], v3 = Qt[s4.w];
+      acc += (double)v0 * w[0];
+      acc += (double)v1 * w[1];
+      acc += (double)v2 * w[2];
+      acc += (double)v3 * w[3];
+    } else {
+      int s16[16];
+      T v[16];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int4 s4 = reinterpret_cast<const int4 *>(src)[l];
+        s16[4 * l] = s4.x; s16[4 * l + 1] = s4.y; s16[4 * l + 2] = s4.z; s16[4 * l + 3] = s4.w;
+      }
+#pragma unroll
+      for (int l = 0; l < 16; ++l) v[l] = Qt[s16[l]];
+#pragma unroll
+      for (int l = 0; l < 16; ++l) acc += (double)v[l] * w[l];
+    }
+    (void)n;
     acc += cf * (strip ? D.strip_gw[c] : D.corner_gw[c]);
     Qt[strip ? D.strip_tgt[c] : D.corner_tgt[c]] = (T)acc;
   }
@@ -586,6 +618,9 @@ struct csadv_s {
   std::vector<int> cur;           // host copy of the current-buffer index per solve
   std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
   int ev_used = 0;
+  int graph_steps = 0;            // steps per captured graph (0 = plain launches)
+  hipStream_t cap = nullptr;      // private stream used only for capture
+  hipGraphExec_t gexec = nullptr; // instantiated graph of graph_steps steps
   std::string err;
 };
 
@@ -642,6 +677,10 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";
     return CSADV_E_ARG;
   }
+  if (((uintptr_t)d.strip_src | (uintptr_t)d.corner_src) % 16 != 0) {
+    msg = "strip_src and corner_src must be 16-byte aligned";
+    return CSADV_E_ARG;
+  }
   if (!(d.dt > 0.0) || !(d.dx > 0.0) || !(d.dy > 0.0)) { msg = "dt, dx, dy must be positive"; return CSADV_E_ARG; }
   return CSADV_OK;
 }
@@ -674,6 +713,11 @@ hipError_t launch_step_dt(csadv_t *h, hipStream_t st) {
     case 4: return launch_step_r<T, 4>(h, st);
     default: return launch_step_r<T, 3>(h, st);
   }
+}
+
+void drop_graph(csadv_t *h) {
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  h->gexec = nullptr;
 }
 
 int step_ghosts(csadv_t *h, hipStream_t st) {
@@ -770,6 +814,7 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
   if (!h) return CSADV_E_ARG;
   if (!solves || nsolve != (int)h->desc.size()) return fail(h, CSADV_E_ARG, "solve count mismatch");
   hipStream_t st = (hipStream_t)stream;
+  drop_graph(h);   // launch shapes may change with the descriptors
   h->host.clear();
   std::vector<int4> work;
   std::vector<FieldSet> sets;
@@ -861,24 +906,79 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
   return ghost_fill(h, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// The kernels of one step, in order, on stream st (event pairs only when armed).
+int enqueue_step(csadv_t *h, hipStream_t st, bool allow_prof) {
+  int rc = wind_avg(h, st, false);
+  if (rc) return rc;
+  rc = step_ghosts(h, st);
+  if (rc) return rc;
+  const bool prof = allow_prof && 2 * (h->ev_used + 1) <= (int)h->ev.size();
+  if (prof) HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used], st));
+  hipError_t e = h->dtype == CSADV_F64 ? launch_step_dt<double>(h, st) : launch_step_dt<float>(h, st);
+  if (e != hipSuccess) return fail(h, CSADV_E_HIP, std::string("step kernel: ") + hipGetErrorString(e));
+  if (prof) {
+    HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used + 1], st));
+    ++h->ev_used;
+  }
+  return CSADV_OK;
+}
+
+// Capture graph_steps steps on the private stream and instantiate them.
+int build_graph(csadv_t *h) {
+  if (!h->cap) HIPCHK(h, hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  HIPCHK(h, hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
+  int rc = CSADV_OK;
+  for (int k = 0; k < h->graph_steps && rc == CSADV_OK; ++k) rc = enqueue_step(h, h->cap, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->cap, &g);
+  if (rc != CSADV_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(h, CSADV_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    h->gexec = nullptr;
+    return fail(h, CSADV_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  }
+  return CSADV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csadv_set_graph(csadv_t *h, int32_t steps_per_graph) {
+  if (!h) return CSADV_E_ARG;
+  if (steps_per_graph < 0 || steps_per_graph > 4096) return fail(h, CSADV_E_RANGE, "steps_per_graph must be in [0, 4096]");
+  if (steps_per_graph != h->graph_steps) drop_graph(h);
+  h->graph_steps = steps_per_graph;
+  return CSADV_OK;
+}
+
 int csadv_step(csadv_t *h, int32_t nsteps, void *stream) {
   if (!h || !h->dev) return CSADV_E_STATE;
   if (nsteps < 0) return fail(h, CSADV_E_ARG, "nsteps < 0");
   hipStream_t st = (hipStream_t)stream;
   const int ns = (int)h->desc.size();
-  for (int k = 0; k < nsteps; ++k) {
-    int rc = wind_avg(h, st, false);
-    if (rc) return rc;
-    rc = step_ghosts(h, st);
-    if (rc) return rc;
-    const bool prof = 2 * (h->ev_used + 1) <= (int)h->ev.size();
-    if (prof) HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used], st));
-    hipError_t e = h->dtype == CSADV_F64 ? launch_step_dt<double>(h, st) : launch_step_dt<float>(h, st);
-    if (e != hipSuccess) return fail(h, CSADV_E_HIP, std::string("step kernel: ") + hipGetErrorString(e));
-    if (prof) {
-      HIPCHK(h, hipEventRecord(h->ev[2 * h->ev_used + 1], st));
-      ++h->ev_used;
+  int k = 0;
+  if (h->graph_steps > 0 && h->ev.empty()) {
+    for (; k + h->graph_steps <= nsteps; k += h->graph_steps) {
+      if (!h->gexec) {
+        const int rc = build_graph(h);
+        if (rc) return rc;
+      }
+      HIPCHK(h, hipGraphLaunch(h->gexec, st));
     }
+  }
+  for (; k < nsteps; ++k) {
+    const int rc = enqueue_step(h, st, true);
+    if (rc) return rc;
   }
   for (int s = 0; s < ns; ++s) h->cur[s] = (h->cur[s] + nsteps) & 1;
   return CSADV_OK;
@@ -990,6 +1090,8 @@ void csadv_destroy(csadv_t *h) {
   if (h->wsets) (void)hipFree(h->wsets);
   if (h->red) (void)hipFree(h->red);
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+  drop_graph(h);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
   delete h;
 }
 

@@ -15,7 +15,7 @@ _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of ord
 # exported symbols, in declaration order of include/csadv.h
 SYMBOLS = (
     "csadv_create", "csadv_partials_per_tracer", "csadv_update", "csadv_wind_init",
-    "csadv_ghost_fill", "csadv_step", "csadv_finalize", "csadv_mass", "csadv_norms",
+    "csadv_ghost_fill", "csadv_step", "csadv_set_graph", "csadv_finalize", "csadv_mass", "csadv_norms",
     "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_last_error", "csadv_destroy",
     "csadv_abi_version",
 )
@@ -66,6 +66,7 @@ def load():
     lib.csadv_wind_init.argtypes = [vp, vp]
     lib.csadv_ghost_fill.argtypes = [vp, vp]
     lib.csadv_step.argtypes = [vp, i32, vp]
+    lib.csadv_set_graph.argtypes = [vp, i32]
     lib.csadv_finalize.argtypes = [vp, vp]
     lib.csadv_mass.argtypes = [vp, i32, i32, dbl, vp]
     lib.csadv_norms.argtypes = [vp, i32, i32, vp, dbl, vp]

@@ -222,6 +222,10 @@ class Advection:
         _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
         self.steps_done += nsteps
 
+    def use_graph(self, steps_per_graph):
+        """Replay groups of ``steps_per_graph`` steps from one captured HIP graph (0 = off)."""
+        _lib.check(self.lib.csadv_set_graph(self.handle, int(steps_per_graph)), self.handle, "csadv_set_graph")
+
     def ghost_fill(self):
         """Duo-grid ghost fill of the current state (src/edges_treatment.py:284-290)."""
         _lib.check(self.lib.csadv_ghost_fill(self.handle, self._stream()), self.handle, "csadv_ghost_fill")
