@@ -93,8 +93,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="copy a buffer of this many MiB device-to-device once before the timed steps "
-                         "(a kernel of known traffic for scaling hardware byte counters)")
+                    help="stream a buffer of this many MiB through one elementwise kernel before the timed "
+                         "steps (a kernel of known traffic for scaling hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -121,7 +121,7 @@ def main():
         n = args.calibrate_copy_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda:0")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)          # one elementwise kernel: reads and writes n * 8 bytes
         torch.cuda.synchronize()
         del src, dst
     sol.step(W)

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "csadv_device.hpp"
@@ -215,13 +216,24 @@ __global__ void k_settle(Solve *solves) {
 // columns (lane l <-> column ti-3+l, so the 3-cell halo of each side rides in the outer
 // lanes) over seg_l interior rows.  Iteration y:
 //   * column stage at y-edge e = y-3 from 6-row register windows (rows y-6 .. y-1): y-fluxes
-//     of Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by lane shuffles), update of
+//     of Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by DPP lane shifts), update of
 //     row y-4 and its mass-fixer partial sum;
-//   * row stage for row y: load (or ghost fill), x-flux of Q, split half-step Qx.
-// The two stages are independent within an iteration.  The last wavefront to finish sums
-// the partials in a fixed order, sets the pending correction, flips the state buffer and
-// advances the step counter (src/discrete_operators.py:18-101, src/advection_timestep.py:43).
-template <typename T, int RECON>
+//   * row stage for row y: x-flux of Q, split half-step Qx.
+// The inputs of row y+1 and y-edge y-2 are requested at the top of iteration y, so their
+// memory latency overlaps the arithmetic of row y; row quantities that the column stage
+// needs four iterations later (metric, 1/g, wind) ride in the same register windows as Q.
+// (src/discrete_operators.py:18-101, src/flux.py, src/reconstruction_1d.py,
+// src/advection_timestep.py:43)
+template <typename T>
+struct RowIn {      // inputs of one row at the lane's column
+  T q, u, us, gu, gc, rg;
+};
+template <typename T>
+struct EdgeIn {     // inputs of one y-edge at the lane's column
+  T v, vs, gvp;     // wind, upwind-side wind, metric of the next y-edge
+};
+
+template <typename T, int RECON, bool RK2>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *work) {
   const int lane = threadIdx.x;
   const int4 wk = work[blockIdx.x];
@@ -242,12 +254,14 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
   const T *U = reinterpret_cast<const T *>(D.uavg) + p * S.pstride_u;
   const T *V = reinterpret_cast<const T *>(D.vavg) + p * S.pstride_v;
-  const T *US = D.usgn ? reinterpret_cast<const T *>(D.usgn) + p * S.pstride_u : U;
-  const T *VS = D.vsgn ? reinterpret_cast<const T *>(D.vsgn) + p * S.pstride_v : V;
+  // RK2 picks the upwind side from separate (un-averaged) winds; RK1 from the winds themselves
+  const T *US = RK2 ? reinterpret_cast<const T *>(D.usgn) + p * S.pstride_u : U;
+  const T *VS = RK2 ? reinterpret_cast<const T *>(D.vsgn) + p * S.pstride_v : V;
   const T *gc = reinterpret_cast<const T *>(D.gc);
   const T *gu = reinterpret_cast<const T *>(D.gu);
   const T *gv = reinterpret_cast<const T *>(D.gv);
   const T *rgc = reinterpret_cast<const T *>(D.rgc);
+  const double *gc64 = D.gc64;
   const T dt = T(D.dt), rdx = T(1.0 / D.dx), rdy = T(1.0 / D.dy);
   const bool mt0 = D.mt == 1;
   const int split = D.split;
@@ -257,53 +271,94 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
     D.ctl[1] = step + 1;
     if (mfpr) D.ctl[3] = 1;
   }
-  const int lm3 = max(lane - 3, 0), lm2 = max(lane - 2, 0), lm1 = max(lane - 1, 0);
-  const int lp1 = min(lane + 1, WAVE - 1), lp2 = min(lane + 2, WAVE - 1);
+  const int ylast = tj + ny + 3;
 
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const T *Qt = Qin + tr * S.tstride;
-      const T *Q = Qt + p * S.pstride;
+    const T *Q = Qin + tr * S.tstride + p * S.pstride;
     T *Qo = Qout + tr * S.tstride + p * S.pstride;
     const double cf = D.corr ? D.corr[tr] : 0.0;
-    T qw[6], xw[6];                 // Q and Qx of rows y-6 .. y-1 of this column
+
+    // row inputs (the q of rows outside the stencil's reach is zero; interior cells carry
+    // the pending mass-fixer correction, ghost cells were filled by k_ghost)
+    auto load_row = [&](int y, RowIn<T> &r) {
+      const int yc = min(y, M - 1);
+      const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
+      r.u = ld_at(U, ou);
+      r.us = RK2 ? ld_at(US, ou) : r.u;
+      r.gu = ld_at(gu, ou);
+      r.gc = ld_at(gc, o);
+      r.rg = ld_at(rgc, o);
+      r.q = T(0);
+      if (need_col && y >= 1 && y < iend + 3) r.q = ld_at(Q, (uint32_t)(y * ld + x));
+    };
+    auto correct = [&](int y, RowIn<T> &r) {
+      if (cf != 0.0 && in_col && y >= NG && y < iend) {
+        const double g = std::is_same<T, double>::value ? (double)r.gc : ld_at(gc64, (uint32_t)(y * ld + x));
+        r.q = (T)((double)r.q + cf * g);
+      }
+    };
+    auto load_edge = [&](int e, EdgeIn<T> &E) {
+      const int ec = min(max(e, 0), M - 1);
+      E.v = ld_at(V, (uint32_t)(ec * ld + xc));
+      E.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : E.v;
+      E.gvp = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
+    };
+
+    // 6-row windows of rows y-6 .. y-1: Q, Qx and the row inputs the column stage reuses
+    T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) { qw[k] = T(0); xw[k] = T(0); }
+    for (int k = 0; k < 6; ++k) { qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = T(0); }
+    // gv at y-edges e-1, e, e+1 (e = y-3), shifted every iteration
+    // (the window is complete from y = tj - 1 on; the column stage starts at y = tj + 3)
+    T gvm1 = T(0), gv0 = T(0), gvp1 = T(0);
     T g1prev = T(0), g2prev = T(0), vprev = T(0);
     double part = 0.0;
 
-    for (int y = tj - 3; y <= tj + ny + 3; ++y) {
+    RowIn<T> cur, nxt;
+    EdgeIn<T> ecur, enxt;
+    load_row(tj - 3, cur);
+    correct(tj - 3, cur);
+    load_edge(tj - 6, ecur);
+
+    for (int y = tj - 3; y <= ylast; ++y) {
+      if (y < ylast) {                         // next iteration's inputs
+        load_row(y + 1, nxt);
+        load_edge(y - 2, enxt);
+      }
+      gvm1 = gv0;
+      gv0 = gvp1;
+      gvp1 = ecur.gvp;
       // ---- column stage: y-edge e = y-3, output row r = y-4
       if (y >= tj + 3) {
-        const int e = y - 3;
-        const T v = V[(int64_t)e * ld + xc];
-        const bool vup = VS[(int64_t)e * ld + xc] >= T(0);
-        const T gvm1 = gv[(int64_t)(e - 1) * ld + xc], gv0 = gv[(int64_t)e * ld + xc], gvp1 = gv[(int64_t)(e + 1) * ld + xc];
-        const T gcm = gc[(int64_t)(e - 1) * ld + xc], gce = gc[(int64_t)e * ld + xc];
+        const T v = ecur.v;
+        const bool vup = ecur.vs >= T(0);
+        const T gcm = gcw[2], gce = gcw[3];   // rows e-1 = y-4 and e = y-3
         const T cy = (v * dt) * rdy;
         const T g1 = edge_flux<T, RECON>(qw[0], qw[1], qw[2], qw[3], qw[4], qw[5], vup, cy, v, gvm1, gv0, gvp1, gcm, gce, mt0);
         const T g2 = edge_flux<T, RECON>(xw[0], xw[1], xw[2], xw[3], xw[4], xw[5], vup, cy, v, gvm1, gv0, gvp1, gcm, gce, mt0);
         if (y >= tj + 4) {
-          const int r = e - 1;
           const T dG1 = (-(g1 - g1prev) * dt) * rdy;
           T d1 = T(0), d2 = T(0);
           if (split != 1) {
             d1 = gv0 * ((v * dt) * rdy);
             d2 = gvm1 * ((vprev * dt) * rdy);
           }
-          const T rg = rgc[(int64_t)r * ld + xc];
+          const T rg = rgw[2];
           const T qy = split_update<T>(split, qw[2], dG1, gcm, rg, d1, d2);
-          const T ur = U[(int64_t)r * ldu + xe];
-          const bool urp = US[(int64_t)r * ldu + xe] >= T(0);
-          const T gur = gu[(int64_t)r * ldu + xe];
-          const T f2 = edge_flux<T, RECON>(lane_shfl(qy, lm3), lane_shfl(qy, lm2), lane_shfl(qy, lm1), qy,
-                                           lane_shfl(qy, lp1), lane_shfl(qy, lp2), urp, (ur * dt) * rdx, ur,
-                                           lane_shfl(gur, lm1), gur, lane_shfl(gur, lp1), lane_shfl(gcm, lm1), gcm, mt0);
-          const T dF2 = (-(lane_shfl(f2, lp1) - f2) * dt) * rdx;
+          const T ur = uw[2];
+          const bool urp = usw[2] >= T(0);
+          const T gur = guw[2];
+          const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
+          const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
+          const T qy_m3 = from_left(qy_m2);
+          const T f2 = edge_flux<T, RECON>(qy_m3, qy_m2, qy_m1, qy, qy_p1, qy_p2, urp, (ur * dt) * rdx, ur,
+                                           from_left(gur), gur, from_right(gur), from_left(gcm), gcm, mt0);
+          const T dF2 = (-(from_right(f2) - f2) * dt) * rdx;
           const T dG2 = (-(g2 - g2prev) * dt) * rdy;
           // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
           const T flx = dF2 + dG2;
           if (out_lane) {
-            Qo[(int64_t)r * ld + x] = qw[2] + flx * rg;
+            st_at(Qo, (uint32_t)((y - 4) * ld + x), qw[2] + flx * rg);
             if (mfpr) part -= (double)flx;
           }
         }
@@ -311,33 +366,34 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
         g2prev = g2;
         vprev = v;
       }
-      // ---- row stage: row y (ghost cells were filled by k_ghost; interior cells carry the
-      //      pending mass-fixer correction)
-      T q = T(0);
-      if (need_col && y >= 1 && y < iend + 3) {
-        q = Q[(int64_t)y * ld + x];
-        if (cf != 0.0 && in_col && y >= NG && y < iend) q = (T)((double)q + cf * D.gc64[(int64_t)y * ld + x]);
-      }
-      const T u = U[(int64_t)y * ldu + xe];
-      const bool up = US[(int64_t)y * ldu + xe] >= T(0);
-      const T gu0 = gu[(int64_t)y * ldu + xe];
-      const T gc0 = gc[(int64_t)y * ld + xc];
-      const T gum1 = lane_shfl(gu0, lm1), gup1 = lane_shfl(gu0, lp1), gcm1 = lane_shfl(gc0, lm1);
-      const T f1 = edge_flux<T, RECON>(lane_shfl(q, lm3), lane_shfl(q, lm2), lane_shfl(q, lm1), q,
-                                       lane_shfl(q, lp1), lane_shfl(q, lp2), up, (u * dt) * rdx, u,
+      // ---- row stage: row y
+      const T q = cur.q, u = cur.u, gu0 = cur.gu, gc0 = cur.gc;
+      const bool up = cur.us >= T(0);
+      const T gum1 = from_left(gu0), gup1 = from_right(gu0), gcm1 = from_left(gc0);
+      const T q_m1 = from_left(q), q_p1 = from_right(q);
+      const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
+      const T q_m3 = from_left(q_m2);
+      const T f1 = edge_flux<T, RECON>(q_m3, q_m2, q_m1, q, q_p1, q_p2, up, (u * dt) * rdx, u,
                                        gum1, gu0, gup1, gcm1, gc0, mt0);
-      const T dF1 = (-(lane_shfl(f1, lp1) - f1) * dt) * rdx;
+      const T dF1 = (-(from_right(f1) - f1) * dt) * rdx;
       T c1 = T(0), c2 = T(0);
       if (split != 1) {
-        const T un = lane_shfl(u, lp1);
+        const T un = from_right(u);
         c1 = gup1 * ((un * dt) * rdx);
         c2 = gu0 * ((u * dt) * rdx);
       }
-      const T qx = split_update<T>(split, q, dF1, gc0, rgc[(int64_t)y * ld + xc], c1, c2);
+      const T qx = split_update<T>(split, q, dF1, gc0, cur.rg, c1, c2);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) { qw[k] = qw[k + 1]; xw[k] = xw[k + 1]; }
-      qw[5] = q;
-      xw[5] = qx;
+      for (int k = 0; k < 5; ++k) {
+        qw[k] = qw[k + 1]; xw[k] = xw[k + 1]; gcw[k] = gcw[k + 1]; rgw[k] = rgw[k + 1];
+        uw[k] = uw[k + 1]; usw[k] = usw[k + 1]; guw[k] = guw[k + 1];
+      }
+      qw[5] = q; xw[5] = qx; gcw[5] = gc0; rgw[5] = cur.rg; uw[5] = u; usw[5] = cur.us; guw[5] = gu0;
+      if (y < ylast) {
+        cur = nxt;
+        correct(y + 1, cur);
+        ecur = enxt;
+      }
     }
     if (mfpr) {
       part = warp_sum(part) / D.dt;           // sum of div g over the wavefront's cells
@@ -614,6 +670,7 @@ struct csadv_s {
   int tile = 0;
   int dtype = 0;
   int recon = 3;
+  bool rk2 = false;               // dp = 2 (separate upwind-side winds)
   double *red = nullptr;          // small reduction scratch (mass / norms)
   std::vector<int> cur;           // host copy of the current-buffer index per solve
   std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
@@ -671,6 +728,7 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "null device pointer in descriptor";
     return CSADV_E_ARG;
   }
+  if (d.dp == 2 && (!d.usgn || !d.vsgn)) { msg = "dp=2 needs the usgn and vsgn wind arrays"; return CSADV_E_ARG; }
   if (d.mf == 3 && (!d.corr || !d.partials)) { msg = "MF-PR needs corr and partials"; return CSADV_E_ARG; }
   if (d.n_strip != 6 * 4 * 3 * d.N || d.n_corner != 6 * 2 * 3 * 6 || !d.strip_tgt || !d.strip_src || !d.strip_w ||
       !d.strip_gw || !d.corner_tgt || !d.corner_src || !d.corner_w || !d.corner_gw) {
@@ -701,7 +759,10 @@ Solve derive(const csadv_solve_desc &d, int tile) {
 
 template <typename T, int R>
 hipError_t launch_step_r(csadv_t *h, hipStream_t st) {
-  hipLaunchKernelGGL((k_step<T, R>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  if (h->rk2)
+    hipLaunchKernelGGL((k_step<T, R, true>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  else
+    hipLaunchKernelGGL((k_step<T, R, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
   return hipGetLastError();
 }
 
@@ -784,12 +845,14 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
   h->tile = tile;
   h->dtype = solves[0].dtype;
   h->recon = solves[0].recon;
+  h->rk2 = solves[0].dp == 2;
   for (int s = 0; s < nsolve; ++s) {
     std::string msg;
     int rc = validate(solves[s], msg);
-    if (rc == CSADV_OK && (solves[s].dtype != h->dtype || solves[s].recon != h->recon)) {
+    if (rc == CSADV_OK && (solves[s].dtype != h->dtype || solves[s].recon != h->recon ||
+                           solves[s].dp != solves[0].dp)) {
       rc = CSADV_E_ARG;
-      msg = "all solves of one handle share dtype and recon";
+      msg = "all solves of one handle share dtype, recon and dp";
     }
     if (rc != CSADV_OK) {
       // the handle is returned so that csadv_last_error() can explain the failure

@@ -159,6 +159,42 @@ __device__ __forceinline__ T lane_shfl(T v, int src) {
   return __shfl(v, src, 64);
 }
 
+// Whole-wavefront DPP lane shifts (one v_mov_b32_dpp per dword, no LDS round trip):
+// from_left(v) is v of lane-1, from_right(v) is v of lane+1; the end lane without a source
+// keeps its own value, exactly as a shuffle from the clamped lane index would.
+constexpr int kDppWaveShl1 = 0x130;   // lane i <- lane i+1
+constexpr int kDppWaveShr1 = 0x138;   // lane i <- lane i-1
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T> __device__ __forceinline__ T from_left(T v) { return dpp_move<kDppWaveShr1>(v); }
+template <typename T> __device__ __forceinline__ T from_right(T v) { return dpp_move<kDppWaveShl1>(v); }
+
+// Element idx of an array whose base is wave-uniform: the byte offset stays 32-bit so that
+// the load takes the scalar-base + vector-offset form (callers keep idx * sizeof(T) < 4 GiB).
+// Every array handed to the library is device global memory, so the access is made in the
+// global address space (a generic pointer would compile to a flat access).
+#define CSADV_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ld_at(const T *base, uint32_t idx) {
+  const char CSADV_GLOBAL *b = (const char CSADV_GLOBAL *)(base);
+  return *(const T CSADV_GLOBAL *)(b + idx * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st_at(T *base, uint32_t idx, T v) {
+  char CSADV_GLOBAL *b = (char CSADV_GLOBAL *)(base);
+  *(T CSADV_GLOBAL *)(b + idx * (uint32_t)sizeof(T)) = v;
+}
+
 // Split update of one cell after the first directional pass (src/discrete_operators.py:49-73).
 //   split 1 AVLT: (gQ + dF/2)/g ; 2 L04: (gQ + dF/2 + (c1-c2) Q/2)/g ; 3 PL07: (Q + (Q+dF)/(1-(c1-c2)))/2
 template <typename T>
