@@ -215,188 +215,210 @@ __global__ void k_settle(Solve *solves) {
 // Work item = (solve, panel, strip, segment): one wavefront marches down strip_w interior
 // columns (lane l <-> column ti-3+l, so the 3-cell halo of each side rides in the outer
 // lanes) over seg_l interior rows.  Iteration y:
-//   * column stage at y-edge e = y-3 from 6-row register windows (rows y-6 .. y-1): y-fluxes
-//     of Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by DPP lane shifts), update of
+//   * column stage at y-edge e = y-3 from the 6-row windows (rows y-6 .. y-1): y-fluxes of
+//     Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by DPP lane shifts), update of
 //     row y-4 and its mass-fixer partial sum;
 //   * row stage for row y: x-flux of Q, split half-step Qx.
-// The inputs of row y+1 and y-edge y-2 are requested at the top of iteration y, so their
-// memory latency overlaps the arithmetic of row y; row quantities that the column stage
-// needs four iterations later (metric, 1/g, wind) ride in the same register windows as Q.
+// The windows are rings indexed by the iteration's phase modulo 6 (the loop is unrolled by
+// six, so every slot is a fixed register and nothing is shifted); row quantities that the
+// column stage needs four iterations later (metric, 1/g, wind) ride in the same rings.
+// The inputs of row y+2 are requested at the end of iteration y into one of two rotating
+// buffers, so their memory latency overlaps a whole iteration of arithmetic.
 // (src/discrete_operators.py:18-101, src/flux.py, src/reconstruction_1d.py,
 // src/advection_timestep.py:43)
 template <typename T>
-struct RowIn {      // inputs of one row at the lane's column
-  T q, u, us, gu, gc, rg;
+struct StepIn {     // inputs of iteration y at the lane's column: row y and y-edge y-3
+  T q, u, us, gu, gc, rg;    // row y: Q, x-edge wind, upwind-side wind, metric, 1/g
+  T v, vs, gvp;              // y-edge y-3: wind, upwind-side wind; metric of y-edge y-2
 };
-template <typename T>
-struct EdgeIn {     // inputs of one y-edge at the lane's column
-  T v, vs, gvp;     // wind, upwind-side wind, metric of the next y-edge
+
+template <typename T, int RECON, bool RK2>
+struct March {
+  // ---- inputs (per wavefront and tracer)
+  const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
+  const double *gc64;
+  T *Qo;
+  int ld, ldu, M, iend, x, xc, xe, tj, ylast, split;
+  bool out_lane, in_col, need_col, mt0, mfpr;
+  T dtdx, dtdy;
+  double cf;
+  // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6)
+  T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6];
+  T gvm1, gv0, gvp1;         // gv at y-edges e-1, e, e+1 (e = y-3)
+  T g1prev, g2prev, vprev;
+  double part;
+
+  __device__ __forceinline__ void load(int y, StepIn<T> &r) const {
+    const int yc = min(y, M - 1);
+    const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
+    r.u = ld_at(U, ou);
+    r.us = RK2 ? ld_at(US, ou) : r.u;
+    r.gu = ld_at(gu, ou);
+    r.gc = ld_at(gc, o);
+    r.rg = ld_at(rgc, o);
+    r.q = T(0);    // rows outside the stencils' reach
+    if (need_col && y >= 1 && y < iend + 3) r.q = ld_at(Q, (uint32_t)(y * ld + x));
+    const int ec = min(max(y - 3, 0), M - 1);
+    r.v = ld_at(V, (uint32_t)(ec * ld + xc));
+    r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
+    r.gvp = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
+  }
+
+  // interior cells carry the pending mass-fixer correction (ghosts got theirs in k_ghost)
+  __device__ __forceinline__ void correct(int y, StepIn<T> &r) const {
+    if (cf != 0.0 && in_col && y >= NG && y < iend) {
+      const double g = std::is_same<T, double>::value ? (double)r.gc : ld_at(gc64, (uint32_t)(y * ld + x));
+      r.q = (T)((double)r.q + cf * g);
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void body(const int y, StepIn<T> &in) {
+    constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
+                  s5 = (K + 5) % 6;
+    correct(y, in);
+    gvm1 = gv0;
+    gv0 = gvp1;
+    gvp1 = in.gvp;
+    // ---- column stage: y-edge e = y-3, output row r = y-4
+    if (y >= tj + 3) {
+      const T v = in.v;
+      const bool vup = in.vs >= T(0);
+      const T gcm = gcw[s2], gce = gcw[s3];   // rows e-1 = y-4 and e = y-3
+      const T cy = v * dtdy;
+      const T g1 = edge_flux<T, RECON>(qw[s0], qw[s1], qw[s2], qw[s3], qw[s4], qw[s5], vup, cy, v, gvm1, gv0, gvp1,
+                                       gcm, gce, mt0);
+      const T g2 = edge_flux<T, RECON>(xw[s0], xw[s1], xw[s2], xw[s3], xw[s4], xw[s5], vup, cy, v, gvm1, gv0, gvp1,
+                                       gcm, gce, mt0);
+      if (y >= tj + 4) {
+        const T dG1 = (g1prev - g1) * dtdy;
+        T d1 = T(0), d2 = T(0);
+        if (split != 1) {
+          d1 = gv0 * cy;
+          d2 = gvm1 * (vprev * dtdy);
+        }
+        const T rg = rgw[s2];
+        const T qy = split_update<T>(split, qw[s2], dG1, gcm, rg, d1, d2);
+        const T ur = uw[s2];
+        const bool urp = usw[s2] >= T(0);
+        const T gur = guw[s2];
+        const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
+        const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
+        const T qy_m3 = from_left(qy_m2);
+        const T f2 = edge_flux<T, RECON>(qy_m3, qy_m2, qy_m1, qy, qy_p1, qy_p2, urp, ur * dtdx, ur, from_left(gur), gur,
+                                         from_right(gur), from_left(gcm), gcm, mt0);
+        const T dF2 = (f2 - from_right(f2)) * dtdx;
+        const T dG2 = (g2prev - g2) * dtdy;
+        // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
+        const T flx = dF2 + dG2;
+        if (out_lane && y <= ylast) {
+          st_at(Qo, (uint32_t)((y - 4) * ld + x), qw[s2] + flx * rg);
+          if (mfpr) part -= (double)flx;
+        }
+      }
+      g1prev = g1;
+      g2prev = g2;
+      vprev = v;
+    }
+    // ---- row stage: row y
+    const T q = in.q, u = in.u, gu0 = in.gu, gc0 = in.gc;
+    const bool up = in.us >= T(0);
+    const T gum1 = from_left(gu0), gup1 = from_right(gu0), gcm1 = from_left(gc0);
+    const T q_m1 = from_left(q), q_p1 = from_right(q);
+    const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
+    const T q_m3 = from_left(q_m2);
+    const T f1 = edge_flux<T, RECON>(q_m3, q_m2, q_m1, q, q_p1, q_p2, up, u * dtdx, u, gum1, gu0, gup1, gcm1, gc0, mt0);
+    const T dF1 = (f1 - from_right(f1)) * dtdx;
+    T c1 = T(0), c2 = T(0);
+    if (split != 1) {
+      c1 = gup1 * (from_right(u) * dtdx);
+      c2 = gu0 * (u * dtdx);
+    }
+    const T qx = split_update<T>(split, q, dF1, gc0, in.rg, c1, c2);
+    // row y replaces row y-6 in every ring
+    qw[s0] = q; xw[s0] = qx; gcw[s0] = gc0; rgw[s0] = in.rg; uw[s0] = u; usw[s0] = in.us; guw[s0] = gu0;
+    if (y + 2 <= ylast) load(y + 2, in);
+  }
+
+  __device__ __forceinline__ void run() {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = T(0);
+    gvm1 = gv0 = gvp1 = T(0);     // complete from y = tj-1 on; the column stage starts at tj+3
+    g1prev = g2prev = vprev = T(0);
+    part = 0.0;
+    StepIn<T> b0, b1;
+    const int y0 = tj - 3;
+    load(y0, b0);
+    load(y0 + 1, b1);
+    int y = y0;
+    for (; y + 5 <= ylast; y += 6) {
+      body<0>(y, b0); body<1>(y + 1, b1); body<2>(y + 2, b0);
+      body<3>(y + 3, b1); body<4>(y + 4, b0); body<5>(y + 5, b1);
+    }
+    if (y <= ylast) body<0>(y, b0);
+    if (y + 1 <= ylast) body<1>(y + 1, b1);
+    if (y + 2 <= ylast) body<2>(y + 2, b0);
+    if (y + 3 <= ylast) body<3>(y + 3, b1);
+    if (y + 4 <= ylast) body<4>(y + 4, b0);
+  }
 };
 
 template <typename T, int RECON, bool RK2>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *work) {
   const int lane = threadIdx.x;
   const int4 wk = work[blockIdx.x];
+  if (wk.x < 0) return;              // padding item of the XCD-banded work list
   const Solve &S = solves[wk.x];
   const csadv_solve_desc &D = S.d;
   const int p = wk.y;
-  const int N = D.N, M = S.M, ld = D.ld, ldu = D.ldu;
-  const int iend = N + NG;
+  const int N = D.N;
   const int ti = NG + wk.z * S.strip_w, tj = NG + wk.w * S.seg_l;
+  const int iend = N + NG;
   const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
-  const int x = ti - 3 + lane;
-  const int xc = min(x, M - 1), xe = min(x, M);          // clamped (lanes past the array)
-  const bool out_lane = lane >= 3 && lane < 3 + nx;
-  const bool in_col = x >= NG && x < iend;
-  const bool need_col = x >= 1 && x < iend + 3;          // columns the stencils reach
   const int step = D.ctl[2];
-  const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
-  T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
-  const T *U = reinterpret_cast<const T *>(D.uavg) + p * S.pstride_u;
-  const T *V = reinterpret_cast<const T *>(D.vavg) + p * S.pstride_v;
-  // RK2 picks the upwind side from separate (un-averaged) winds; RK1 from the winds themselves
-  const T *US = RK2 ? reinterpret_cast<const T *>(D.usgn) + p * S.pstride_u : U;
-  const T *VS = RK2 ? reinterpret_cast<const T *>(D.vsgn) + p * S.pstride_v : V;
-  const T *gc = reinterpret_cast<const T *>(D.gc);
-  const T *gu = reinterpret_cast<const T *>(D.gu);
-  const T *gv = reinterpret_cast<const T *>(D.gv);
-  const T *rgc = reinterpret_cast<const T *>(D.rgc);
-  const double *gc64 = D.gc64;
-  const T dt = T(D.dt), rdx = T(1.0 / D.dx), rdy = T(1.0 / D.dy);
-  const bool mt0 = D.mt == 1;
-  const int split = D.split;
-  const bool mfpr = D.mf == 3;
   const int item = (p * S.segs + wk.w) * S.strips + wk.z;
+  const bool mfpr = D.mf == 3;
   if (item == 0 && lane == 0) {      // hand the step over to the next kernels of the stream
     D.ctl[1] = step + 1;
     if (mfpr) D.ctl[3] = 1;
   }
-  const int ylast = tj + ny + 3;
-
+  March<T, RECON, RK2> m;
+  m.ld = D.ld;
+  m.ldu = D.ldu;
+  m.M = S.M;
+  m.iend = iend;
+  m.x = ti - 3 + lane;
+  m.xc = min(m.x, m.M - 1);          // clamped (lanes past the array)
+  m.xe = min(m.x, m.M);
+  m.tj = tj;
+  m.ylast = tj + ny + 3;
+  m.split = D.split;
+  m.out_lane = lane >= 3 && lane < 3 + nx;
+  m.in_col = m.x >= NG && m.x < iend;
+  m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
+  m.mt0 = D.mt == 1;
+  m.mfpr = mfpr;
+  m.dtdx = T(D.dt / D.dx);
+  m.dtdy = T(D.dt / D.dy);
+  m.U = reinterpret_cast<const T *>(D.uavg) + p * S.pstride_u;
+  m.V = reinterpret_cast<const T *>(D.vavg) + p * S.pstride_v;
+  // RK2 picks the upwind side from separate (un-averaged) winds; RK1 from the winds themselves
+  m.US = RK2 ? reinterpret_cast<const T *>(D.usgn) + p * S.pstride_u : m.U;
+  m.VS = RK2 ? reinterpret_cast<const T *>(D.vsgn) + p * S.pstride_v : m.V;
+  m.gc = reinterpret_cast<const T *>(D.gc);
+  m.gu = reinterpret_cast<const T *>(D.gu);
+  m.gv = reinterpret_cast<const T *>(D.gv);
+  m.rgc = reinterpret_cast<const T *>(D.rgc);
+  m.gc64 = D.gc64;
+  const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
+  T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const T *Q = Qin + tr * S.tstride + p * S.pstride;
-    T *Qo = Qout + tr * S.tstride + p * S.pstride;
-    const double cf = D.corr ? D.corr[tr] : 0.0;
-
-    // row inputs (the q of rows outside the stencil's reach is zero; interior cells carry
-    // the pending mass-fixer correction, ghost cells were filled by k_ghost)
-    auto load_row = [&](int y, RowIn<T> &r) {
-      const int yc = min(y, M - 1);
-      const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
-      r.u = ld_at(U, ou);
-      r.us = RK2 ? ld_at(US, ou) : r.u;
-      r.gu = ld_at(gu, ou);
-      r.gc = ld_at(gc, o);
-      r.rg = ld_at(rgc, o);
-      r.q = T(0);
-      if (need_col && y >= 1 && y < iend + 3) r.q = ld_at(Q, (uint32_t)(y * ld + x));
-    };
-    auto correct = [&](int y, RowIn<T> &r) {
-      if (cf != 0.0 && in_col && y >= NG && y < iend) {
-        const double g = std::is_same<T, double>::value ? (double)r.gc : ld_at(gc64, (uint32_t)(y * ld + x));
-        r.q = (T)((double)r.q + cf * g);
-      }
-    };
-    auto load_edge = [&](int e, EdgeIn<T> &E) {
-      const int ec = min(max(e, 0), M - 1);
-      E.v = ld_at(V, (uint32_t)(ec * ld + xc));
-      E.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : E.v;
-      E.gvp = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
-    };
-
-    // 6-row windows of rows y-6 .. y-1: Q, Qx and the row inputs the column stage reuses
-    T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) { qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = T(0); }
-    // gv at y-edges e-1, e, e+1 (e = y-3), shifted every iteration
-    // (the window is complete from y = tj - 1 on; the column stage starts at y = tj + 3)
-    T gvm1 = T(0), gv0 = T(0), gvp1 = T(0);
-    T g1prev = T(0), g2prev = T(0), vprev = T(0);
-    double part = 0.0;
-
-    RowIn<T> cur, nxt;
-    EdgeIn<T> ecur, enxt;
-    load_row(tj - 3, cur);
-    correct(tj - 3, cur);
-    load_edge(tj - 6, ecur);
-
-    for (int y = tj - 3; y <= ylast; ++y) {
-      if (y < ylast) {                         // next iteration's inputs
-        load_row(y + 1, nxt);
-        load_edge(y - 2, enxt);
-      }
-      gvm1 = gv0;
-      gv0 = gvp1;
-      gvp1 = ecur.gvp;
-      // ---- column stage: y-edge e = y-3, output row r = y-4
-      if (y >= tj + 3) {
-        const T v = ecur.v;
-        const bool vup = ecur.vs >= T(0);
-        const T gcm = gcw[2], gce = gcw[3];   // rows e-1 = y-4 and e = y-3
-        const T cy = (v * dt) * rdy;
-        const T g1 = edge_flux<T, RECON>(qw[0], qw[1], qw[2], qw[3], qw[4], qw[5], vup, cy, v, gvm1, gv0, gvp1, gcm, gce, mt0);
-        const T g2 = edge_flux<T, RECON>(xw[0], xw[1], xw[2], xw[3], xw[4], xw[5], vup, cy, v, gvm1, gv0, gvp1, gcm, gce, mt0);
-        if (y >= tj + 4) {
-          const T dG1 = (-(g1 - g1prev) * dt) * rdy;
-          T d1 = T(0), d2 = T(0);
-          if (split != 1) {
-            d1 = gv0 * ((v * dt) * rdy);
-            d2 = gvm1 * ((vprev * dt) * rdy);
-          }
-          const T rg = rgw[2];
-          const T qy = split_update<T>(split, qw[2], dG1, gcm, rg, d1, d2);
-          const T ur = uw[2];
-          const bool urp = usw[2] >= T(0);
-          const T gur = guw[2];
-          const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
-          const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
-          const T qy_m3 = from_left(qy_m2);
-          const T f2 = edge_flux<T, RECON>(qy_m3, qy_m2, qy_m1, qy, qy_p1, qy_p2, urp, (ur * dt) * rdx, ur,
-                                           from_left(gur), gur, from_right(gur), from_left(gcm), gcm, mt0);
-          const T dF2 = (-(from_right(f2) - f2) * dt) * rdx;
-          const T dG2 = (-(g2 - g2prev) * dt) * rdy;
-          // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
-          const T flx = dF2 + dG2;
-          if (out_lane) {
-            st_at(Qo, (uint32_t)((y - 4) * ld + x), qw[2] + flx * rg);
-            if (mfpr) part -= (double)flx;
-          }
-        }
-        g1prev = g1;
-        g2prev = g2;
-        vprev = v;
-      }
-      // ---- row stage: row y
-      const T q = cur.q, u = cur.u, gu0 = cur.gu, gc0 = cur.gc;
-      const bool up = cur.us >= T(0);
-      const T gum1 = from_left(gu0), gup1 = from_right(gu0), gcm1 = from_left(gc0);
-      const T q_m1 = from_left(q), q_p1 = from_right(q);
-      const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
-      const T q_m3 = from_left(q_m2);
-      const T f1 = edge_flux<T, RECON>(q_m3, q_m2, q_m1, q, q_p1, q_p2, up, (u * dt) * rdx, u,
-                                       gum1, gu0, gup1, gcm1, gc0, mt0);
-      const T dF1 = (-(from_right(f1) - f1) * dt) * rdx;
-      T c1 = T(0), c2 = T(0);
-      if (split != 1) {
-        const T un = from_right(u);
-        c1 = gup1 * ((un * dt) * rdx);
-        c2 = gu0 * ((u * dt) * rdx);
-      }
-      const T qx = split_update<T>(split, q, dF1, gc0, cur.rg, c1, c2);
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        qw[k] = qw[k + 1]; xw[k] = xw[k + 1]; gcw[k] = gcw[k + 1]; rgw[k] = rgw[k + 1];
-        uw[k] = uw[k + 1]; usw[k] = usw[k + 1]; guw[k] = guw[k + 1];
-      }
-      qw[5] = q; xw[5] = qx; gcw[5] = gc0; rgw[5] = cur.rg; uw[5] = u; usw[5] = cur.us; guw[5] = gu0;
-      if (y < ylast) {
-        cur = nxt;
-        correct(y + 1, cur);
-        ecur = enxt;
-      }
-    }
+    m.Q = Qin + tr * S.tstride + p * S.pstride;
+    m.Qo = Qout + tr * S.tstride + p * S.pstride;
+    m.cf = D.corr ? D.corr[tr] : 0.0;
+    m.run();
     if (mfpr) {
-      part = warp_sum(part) / D.dt;           // sum of div g over the wavefront's cells
+      const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
       if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
     }
   }
@@ -694,6 +716,7 @@ void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs) {
   segs = (N + seg_l - 1) / seg_l;
 }
 constexpr int kRedBlocks = 256;
+constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 int fail(csadv_t *h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -879,7 +902,7 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
   hipStream_t st = (hipStream_t)stream;
   drop_graph(h);   // launch shapes may change with the descriptors
   h->host.clear();
-  std::vector<int4> work;
+  std::vector<std::vector<int4>> band(kXcds);
   std::vector<FieldSet> sets;
   for (int s = 0; s < nsolve; ++s) {
     std::string msg;
@@ -888,9 +911,11 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
     h->desc[s] = solves[s];
     Solve S = derive(solves[s], h->tile);
     h->host.push_back(S);
-    for (int p = 0; p < 6; ++p)
-      for (int sg = 0; sg < S.segs; ++sg)
-        for (int sx = 0; sx < S.strips; ++sx) work.push_back(make_int4(s, p, sx, sg));
+    // segment rows are cut into kXcds contiguous bands, one per XCD: the band's halo rows and
+    // its share of the panel-invariant metric then stay in that XCD's L2 for all six panels
+    for (int sg = 0; sg < S.segs; ++sg)
+      for (int p = 0; p < 6; ++p)
+        for (int sx = 0; sx < S.strips; ++sx) band[sg * kXcds / S.segs].push_back(make_int4(s, p, sx, sg));
     FieldSet F{};
     F.base[0] = solves[s].q0;
     F.base[1] = solves[s].q1;
@@ -908,6 +933,13 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
     F.orient = solves[s].orient;
     sets.push_back(F);
   }
+  // workgroup b runs on XCD b % kXcds: interleave the bands, padding them to equal length
+  size_t blen = 0;
+  for (auto &bd : band) blen = std::max(blen, bd.size());
+  std::vector<int4> work;
+  work.reserve(blen * kXcds);
+  for (size_t i = 0; i < blen; ++i)
+    for (int x = 0; x < kXcds; ++x) work.push_back(i < band[x].size() ? band[x][i] : make_int4(-1, 0, 0, 0));
   if ((int)work.size() != h->nwork) {
     if (h->work) (void)hipFree(h->work);
     h->work = nullptr;

@@ -161,19 +161,20 @@ __device__ __forceinline__ T lane_shfl(T v, int src) {
 
 // Whole-wavefront DPP lane shifts (one v_mov_b32_dpp per dword, no LDS round trip):
 // from_left(v) is v of lane-1, from_right(v) is v of lane+1; the end lane without a source
-// keeps its own value, exactly as a shuffle from the clamped lane index would.
+// (lane 0, resp. lane 63) reads zero.  The step kernel only ever stores results of lanes
+// whose stencils stay at least two lanes away from either end, so the end values never
+// reach a stored cell.
 constexpr int kDppWaveShl1 = 0x130;   // lane i <- lane i+1
 constexpr int kDppWaveShr1 = 0x138;   // lane i <- lane i-1
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(float v) {
-  const int b = __float_as_int(v);
-  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 template <typename T> __device__ __forceinline__ T from_left(T v) { return dpp_move<kDppWaveShr1>(v); }
