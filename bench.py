@@ -1,23 +1,36 @@
 #!/usr/bin/env python
 """Benchmark of the cubed-sphere advection step on one MI355X.
 
-Default workload (N=1): the plain FP64 GPU run -- equiangular cubed sphere N=384
-(6x384x384 cells), one tracer of two Gaussian hills (b=5) at lon 5pi/6 and 7pi/6 on
-the equator, solid-body rotation about an axis tilted pi/4 from the pole, CFL 0.5,
-1000 steps.  A "step" is one full advection step of every cell: ghost fill, both
-directional passes of the split, update and mass fixer.
+Workloads (--workload):
+  two_hills_n384_f64   default; the plain FP64 GPU run: equiangular cubed sphere N=384
+                       (6x384x384 cells), one tracer of two Gaussian hills (b=5) at lon
+                       5pi/6 and 7pi/6 on the equator, solid-body rotation about an axis
+                       tilted pi/4 from the pole, CFL 0.5, 1000 steps
+  nl10_n768_f32        precision form: N=768, 4 tracers (Gaussian hills, cosine bells,
+                       slotted cylinders, their mix) in the Nair-Lauritzen deformational
+                       wind (k=2.4, T=5), FP32 state and fluxes, FP64 tables and mass
+  batched_64_f32       batched form: 64 seeded solves, N in {32,48,96,192,384,512}, 1-8
+                       tracers, random Gaussian centre / width / rotation-axis tilt, FP32,
+                       one launch per kernel per step over all solves, graph-replayed,
+                       tile shape from the autotuner
+  polar_n48_f64        the reference's CPU-sized case (N=48 polar rotation), for reference
+
+A "step" is one full advection step of every cell of every solve: ghost fill, both
+directional passes of the split, update and mass fixer (plus the wind update of
+time-dependent winds).
 
 Prints ONE JSON line:
   value        tracer cell-updates/s over the timed steps (inputs resident in HBM)
   roofline     the step kernel against the HBM roofline (algorithmic bytes per launch
-               over its event-timed mean duration); traffic from the committed
-               rocprofv3 counter summary when present
+               over its event-timed mean duration); traffic from the committed rocprofv3
+               counter summary of the workload when present
   cpu_baseline the numpy FP64 oracle on this host, one core, on a bounded sample of the
-               same workload (a few steps), scaled to cell-updates/s
+               same workload (a few steps of one solve), scaled to cell-updates/s
+  l2_vs_f64    (FP32 workloads) relative L2 difference of the final states from an FP64
+               GPU run of the same solves and steps
 """
 import argparse
 import json
-import math
 import os
 import re
 import sys
@@ -31,11 +44,17 @@ for _p in (ROOT, os.path.join(ROOT, "py-cubed-sphere_amd")):
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, published
 
 WORKLOADS = {
-    # name: (N, case name, dtype, cfl, default steps)
-    "two_hills_n384_f64": (384, "two_hills_tilted", "f64", 0.5, 1000),
-    "nl10_n768_f32": (768, "nl10_deformation", "f32", 0.5, 200),
-    "polar_n48_f64": (48, "solid_body_polar", "f64", 0.5, 384),
+    # name: (kind, N, case name, dtype, cfl, default steps)
+    "two_hills_n384_f64": ("single", 384, "two_hills_tilted", "f64", 0.5, 1000),
+    "nl10_n768_f32": ("single", 768, "nl10_deformation", "f32", 0.5, 300),
+    "batched_64_f32": ("batched", None, None, "f32", 0.5, 200),
+    "polar_n48_f64": ("single", 48, "solid_body_polar", "f64", 0.5, 384),
 }
+BATCH_NS = (32, 48, 96, 192, 384, 512)
+# tile ids from the autotuner for the single-solve workloads (the batched workload tunes
+# itself at start-up over BATCH_TILE_CANDIDATES)
+TUNED_TILE = {("f64", 384): 0, ("f32", 768): 0, ("f64", 48): 1}
+BATCH_TILE_CANDIDATES = (0, 2, 6, 8, 9, 10)
 
 
 def algorithmic_bytes_per_update(dtype, ntr):
@@ -56,8 +75,8 @@ def traffic_from_profile(name):
 
 
 def cpu_baseline(N, case, dt, seconds):
-    """Time the oracle (numpy FP64, single thread) on the same workload: 1 warm-up step,
-    then as many steps as fit in about ``seconds``."""
+    """Time the oracle (numpy FP64, single thread) on one solve of the workload: 1 warm-up
+    step, then as many steps as fit in about ``seconds``."""
     from oracle import timestep as ots
     from oracle.cases import from_case
     grid, sim = ots.make(N, from_case(case), dt)
@@ -76,8 +95,57 @@ def cpu_baseline(N, case, dt, seconds):
         "unit": "tracer cell-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": "numpy FP64 oracle, N=%d, %d timed steps after 1 warm-up (%.1f s, %.3f s/step)" % (N, steps, el, per_step),
+        "sample": "numpy FP64 oracle, N=%d, %d tracer(s), %d timed steps after 1 warm-up (%.1f s, %.3f s/step)"
+                  % (N, case.ntr, steps, el, per_step),
     }
+
+
+def batch_specs(seed, steps):
+    """The 64 seeded solves of the batched form: (N, case, steps)."""
+    import numpy as np
+    from cubedsphere.cases import random_solid_body
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(64):
+        N = int(rng.choice(BATCH_NS))
+        ntr = int(rng.integers(1, 9))
+        specs.append((N, random_solid_body(rng, ntr), steps))
+    return specs
+
+
+def timed_run(stepper, K, W, graph):
+    """W warm-up steps, then K timed steps (graph replay when graph > 0, else plain launches
+    with event pairs around each step-kernel launch); returns (elapsed s, kernel ms, launches)."""
+    import torch
+    stepper.step(W)
+    if graph > 0:
+        stepper.use_graph(graph)
+        stepper.step(graph)                  # capture outside the timed region
+    else:
+        stepper.profile(K)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stepper.step(K)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if graph > 0:                            # separate plain-launch pass for the kernel duration
+        stepper.use_graph(0)
+        stepper.profile(K)
+        stepper.step(K)
+    kern_ms, launches = stepper.profile_read()
+    stepper.profile(0)
+    return t1 - t0, kern_ms, launches
+
+
+def l2_vs_f64(members32, members64):
+    """max over solves and tracers of ||q32 - q64|| / ||q64|| (interiors)."""
+    import numpy as np
+    worst = 0.0
+    for a, b in zip(members32, members64):
+        qa, qb = a.state(), b.state()
+        for n in range(qa.shape[0]):
+            worst = max(worst, float(np.linalg.norm(qa[n] - qb[n]) / np.linalg.norm(qb[n])))
+    return worst
 
 
 def main():
@@ -86,12 +154,14 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="two_hills_n384_f64", choices=sorted(WORKLOADS))
-    ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--graph", type=int, default=0,
+    ap.add_argument("--tile", type=int, default=None, help="tile id (default: tuned for the workload)")
+    ap.add_argument("--graph", type=int, default=16,
                     help="replay the timed steps from HIP graphs of this many steps (0 = plain launches); "
                          "the step-kernel duration then comes from a separate plain-launch pass")
+    ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-l2", action="store_true", help="skip the FP64 comparison run of FP32 workloads")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="stream a buffer of this many MiB through one elementwise kernel before the timed "
                          "steps (a kernel of known traffic for scaling hardware byte counters)")
@@ -104,19 +174,28 @@ def main():
 
     import torch
 
+    from cubedsphere.autotune import tile_rows, tune_batch
+    from cubedsphere.batch import Batch, make_members
     from cubedsphere.cases import workload
-    from cubedsphere.grid import Grid
-    from cubedsphere.solver import Advection, cfl_dt
 
-    N, case_name, dtype, cfl, default_steps = WORKLOADS[args.workload]
+    kind, N, case_name, dtype, cfl, default_steps = WORKLOADS[args.workload]
     K = args.steps if args.steps is not None else default_steps
     W = args.warmup
-    case = workload(case_name)
-    grid = Grid(N)
-    dt = cfl_dt(grid, case, cfl)
-    sol = Advection(N, case, dt, dtype=dtype, device="cuda:0", tile=args.tile,
-                    max_steps=W + 2 * K + max(args.graph, 0), grid=grid)
-    sol.init_vars_adv()
+    nstep_total = W + 2 * K + max(args.graph, 0)
+    tuned = None
+    if kind == "single":
+        specs = [(N, workload(case_name), nstep_total)]
+        tile = args.tile if args.tile is not None else TUNED_TILE.get((dtype, N), 0)
+    else:
+        specs = batch_specs(args.seed, nstep_total)
+        if args.tile is not None:
+            tile = args.tile
+        else:
+            tile, timings = tune_batch(specs, dtype=dtype, candidates=BATCH_TILE_CANDIDATES, steps=16, warmup=8)
+            tuned = {str(t): round(v * 1e6, 1) for t, v in timings.items()}
+    tiles = {s[0]: tile for s in specs}
+    members = make_members(specs, dtype=dtype, cfl=cfl, tiles=tiles)
+    stepper = Batch(members, graph_steps=0).init()
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda:0")
@@ -124,32 +203,16 @@ def main():
         torch.mul(src, 2.0, out=dst)          # one elementwise kernel: reads and writes n * 8 bytes
         torch.cuda.synchronize()
         del src, dst
-    sol.step(W)
-    if args.graph > 0:
-        sol.use_graph(args.graph)
-        sol.step(args.graph)                 # capture outside the timed region
-    else:
-        sol.profile(K)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sol.step(K)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    if args.graph > 0:
-        sol.use_graph(0)
-        sol.profile(K)
-        sol.step(K)
-    kern_ms, launches = sol.profile_read()
-    sol.profile(0)
 
-    updates = 6.0 * N * N * case.ntr * K
-    elapsed = t1 - t0
-    value = updates / elapsed
-    b_alg = algorithmic_bytes_per_update(dtype, case.ntr)
-    bytes_per_launch = b_alg * 6.0 * N * N * case.ntr
+    elapsed, kern_ms, launches = timed_run(stepper, K, W, args.graph)
+    per_step = stepper.cell_updates_per_step()
+    value = per_step * K / elapsed
+    bytes_per_launch = sum(algorithmic_bytes_per_update(dtype, m.case.ntr) * 6.0 * m.grid.N ** 2 * m.case.ntr
+                           for m in members)
+    b_alg = bytes_per_launch / per_step
     kern_s = kern_ms / 1e3 / max(launches, 1)
     achieved = bytes_per_launch / kern_s / 1e9
-    traffic = traffic_from_profile(args.workload)
+    Ns = sorted({m.grid.N for m in members})
 
     out = {
         "metric": "tracer cell-updates/s",
@@ -163,9 +226,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
-        "data": "synthetic (analytic initial condition and wind on the generated grid)",
-        "config": {"workload": args.workload, "N": N, "cells": 6 * N * N, "tracers": case.ntr,
-                   "case": case_name, "cfl": cfl, "dt": dt, "tile": args.tile,
+        "data": "synthetic (analytic initial conditions and winds on the generated grids)",
+        "config": {"workload": args.workload, "solves": len(members), "N": Ns if len(Ns) > 1 else Ns[0],
+                   "cells_per_step": per_step, "tracers": sum(m.case.ntr for m in members), "cfl": cfl,
+                   "tile": tile, "tile_rows": tile_rows(tile), "graph_steps": args.graph,
                    "options": "recon PPM-PL07, dp RK1, split AVLT, ET-DG, MT-0, MF-PR"},
         "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
         "roofline": {
@@ -174,15 +238,27 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": traffic_from_profile(args.workload),
             "kernel": "k_step",
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "mean_launch_us": kern_s * 1e6,
             "launches_timed": launches,
         },
     }
+    if tuned is not None:
+        out["config"]["autotune_us_per_step"] = tuned
+    if dtype == "f32" and not args.no_l2:
+        # the same solves in FP64 for as many steps as the FP32 run took
+        done = members[0].steps_done
+        m64 = make_members([(s[0], s[1], done) for s in specs], dtype="f64", cfl=cfl, tiles=tiles)
+        b64 = Batch(m64, graph_steps=16).init()
+        b64.step(done)
+        out["l2_vs_f64"] = l2_vs_f64(members, m64)
+        out["l2_vs_f64_steps"] = done
+        b64.close()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(N, case, dt, args.cpu_seconds)
+        m0 = members[0]
+        out["cpu_baseline"] = cpu_baseline(m0.grid.N, m0.case, m0.dt, args.cpu_seconds)
     print(json.dumps(out))
 
 

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CSADV_ABI_VERSION 1
+#define CSADV_ABI_VERSION 2
 
 enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
 
@@ -83,6 +83,11 @@ typedef struct csadv_solve_desc {
   double *corr;                    /* [ntr] pending mass-fixer correction (FP64)        */
   double *partials;                /* scratch, >= ntr * csadv_partials_per_tracer()     */
   int32_t *ctl;                    /* [4] device control words: cur, step, 0, 0          */
+  int32_t tile;                    /* step tile shape of this solve (csadv_tile_rows);
+                                      -1 = the handle's tile.  The per-wavefront mass-fixer
+                                      partial sums follow the tile, so a solve keeps the same
+                                      bits alone and in any batch when its tile is fixed here */
+  int32_t pad_;
 } csadv_solve_desc;
 
 typedef struct csadv_s csadv_t;
@@ -95,6 +100,10 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
 
 /* Number of FP64 partial sums the step writes per tracer (size of desc.partials / ntr). */
 int csadv_partials_per_tracer(int32_t N, int32_t tile);
+
+/* Rows per wavefront segment of a tile shape (tile = 0 .. count-1), or CSADV_E_ARG past the
+ * last shape: the tile ids an autotuner may try.  No counterpart in the reference. */
+int csadv_tile_rows(int32_t tile);
 
 /* Refresh the device copy of the descriptors after the caller changed pointers. */
 int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, void *stream);
@@ -122,9 +131,11 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream);
  * and releases the graph.  No counterpart in the reference (its step is a Python call). */
 int csadv_set_graph(csadv_t *h, int32_t steps_per_graph);
 
-/* Fold the pending mass-fixer corrections into the current state buffer so that it
- * holds Q exactly as the reference would after the last step. */
-int csadv_finalize(csadv_t *h, void *stream);
+/* Fold the pending mass-fixer corrections of one solve (solve = -1: every solve) into its
+ * current state buffer so that it holds Q exactly as the reference would after the last
+ * step.  Other solves of the handle are left untouched, so reading one result never
+ * changes the arithmetic of the solves still running. */
+int csadv_finalize(csadv_t *h, int32_t solve, void *stream);
 
 /* Total mass sum_int Q g dx dy of one tracer with a compensated FP64 sum
  * (src/diagnostics.py:14-26).  Synchronises the stream and writes host out[0]. */

@@ -160,38 +160,40 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
   const bool active = idx < ncell;
   const bool strip = idx < D.n_strip;
   const int c = strip ? idx : idx - D.n_strip;
-  const int n = strip ? 4 : 16;
   const int32_t *src = strip ? D.strip_src + 4 * c : D.corner_src + 16 * c;
   const double *w = strip ? D.strip_w + 4 * c : D.corner_w + 16 * c;
+  // source offsets are the same for every tracer
+  int s16[16];
+  if (active) {
+    const int nq = strip ? 1 : 4;
+    for (int l = 0; l < nq; ++l) {
+      const int4 s4 = reinterpret_cast<const int4 *>(src)[l];
+      s16[4 * l] = s4.x; s16[4 * l + 1] = s4.y; s16[4 * l + 2] = s4.z; s16[4 * l + 3] = s4.w;
+    }
+  }
   for (int tr = 0; tr < D.ntr; ++tr) {
+    T *Qt = Q + tr * S.tstride;
+    // weighted sum of the sources first (in recipe order), so that its loads overlap the
+    // block-wide reduction of the mass-fixer partials below
+    double acc = 0.0;
+    if (active) {
+      if (strip) {
+        const T v0 = Qt[s16[0]], v1 = Qt[s16[1]], v2 = Qt[s16[2]], v3 = Qt[s16[3]];
+        acc += (double)v0 * w[0];
+        acc += (double)v1 * w[1];
+        acc += (double)v2 * w[2];
+        acc += (double)v3 * w[3];
+      } else {
+        T v[16];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) v[l] = Qt[s16[l]];
+#pragma unroll
+        for (int l = 0; l < 16; ++l) acc += (double)v[l] * w[l];
+      }
+    }
     const double cf = pending_corr(S, tr, red);
     if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && D.ctl[3]) D.corr[tr] = cf;
     if (!active) continue;
-    T *Qt = Q + tr * S.tstride;
-    double acc = 0.0;
-    // all source indices, then all source values, are requested before the first use;
-    // the sum runs in recipe order either way
-    if (strip) {
-      const int4 s4 = *reinterpret_cast<const int4 *>(src);
-      const T v0 = Qt[s4.x], v1 = Qt[s4.y], v2 = Qt[s4.z], v3 = Qt[s4.w];
-      acc += (double)v0 * w[0];
-      acc += (double)v1 * w[1];
-      acc += (double)v2 * w[2];
-      acc += (double)v3 * w[3];
-    } else {
-      int s16[16];
-      T v[16];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const int4 s4 = reinterpret_cast<const int4 *>(src)[l];
-        s16[4 * l] = s4.x; s16[4 * l + 1] = s4.y; s16[4 * l + 2] = s4.z; s16[4 * l + 3] = s4.w;
-      }
-#pragma unroll
-      for (int l = 0; l < 16; ++l) v[l] = Qt[s16[l]];
-#pragma unroll
-      for (int l = 0; l < 16; ++l) acc += (double)v[l] * w[l];
-    }
-    (void)n;
     acc += cf * (strip ? D.strip_gw[c] : D.corner_gw[c]);
     Qt[strip ? D.strip_tgt[c] : D.corner_tgt[c]] = (T)acc;
   }
@@ -199,9 +201,9 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
 }
 
 // Fold the last step's partials into corr (once; used by finalize and the diagnostics).
-__global__ void k_settle(Solve *solves) {
+__global__ void k_settle(Solve *solves, int first) {
   __shared__ double red[BLOCK];
-  Solve &S = solves[blockIdx.x];
+  Solve &S = solves[first + blockIdx.x];
   csadv_solve_desc &D = S.d;
   if (D.mf != 3 || !D.ctl[3]) return;
   for (int tr = 0; tr < D.ntr; ++tr) {
@@ -426,8 +428,8 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
 
 // Fold pending corrections into the current buffer (interior only) and clear them.
 template <typename T>
-__global__ void k_finalize(Solve *solves) {
-  Solve &S = solves[blockIdx.y];
+__global__ void k_finalize(Solve *solves, int first) {
+  Solve &S = solves[first + blockIdx.y];
   csadv_solve_desc &D = S.d;
   if (D.mf != 3) return;
   const int N = D.N, ld = D.ld;
@@ -446,8 +448,8 @@ __global__ void k_finalize(Solve *solves) {
   }
 }
 
-__global__ void k_clear_corr(Solve *solves) {
-  csadv_solve_desc &D = solves[blockIdx.x].d;
+__global__ void k_clear_corr(Solve *solves, int first) {
+  csadv_solve_desc &D = solves[first + blockIdx.x].d;
   if (D.mf == 3)
     for (int tr = threadIdx.x; tr < D.ntr; tr += blockDim.x) D.corr[tr] = 0.0;
 }
@@ -706,7 +708,7 @@ struct csadv_s {
 namespace {
 
 // tile id -> rows per wavefront segment (the strip width is always the widest that fits)
-constexpr int kSegRows[] = {8, 4, 16, 32, 6, 12};
+constexpr int kSegRows[] = {8, 4, 16, 32, 6, 12, 11, 5, 10, 14, 17, 23};
 constexpr int kNumTiles = sizeof(kSegRows) / sizeof(kSegRows[0]);
 
 void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs) {
@@ -762,6 +764,7 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "strip_src and corner_src must be 16-byte aligned";
     return CSADV_E_ARG;
   }
+  if (d.tile < -1 || d.tile >= kNumTiles) { msg = "tile must be -1 or a valid tile id"; return CSADV_E_RANGE; }
   if (!(d.dt > 0.0) || !(d.dx > 0.0) || !(d.dy > 0.0)) { msg = "dt, dx, dy must be positive"; return CSADV_E_ARG; }
   return CSADV_OK;
 }
@@ -770,7 +773,7 @@ Solve derive(const csadv_solve_desc &d, int tile) {
   Solve s{};
   s.d = d;
   s.M = d.N + 2 * NG;
-  tiling(d.N, tile, s.strip_w, s.strips, s.seg_l, s.segs);
+  tiling(d.N, d.tile >= 0 ? d.tile : tile, s.strip_w, s.strips, s.seg_l, s.segs);
   s.items = s.strips * s.segs;
   s.pstride = (int64_t)s.M * d.ld;
   s.tstride = 6 * s.pstride;
@@ -857,6 +860,11 @@ int csadv_partials_per_tracer(int32_t N, int32_t tile) {
   int sw, ns, sl, nsg;
   tiling(N, tile, sw, ns, sl, nsg);
   return 6 * ns * nsg;
+}
+
+int csadv_tile_rows(int32_t tile) {
+  if (tile < 0 || tile >= kNumTiles) return CSADV_E_ARG;
+  return kSegRows[tile];
 }
 
 int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, csadv_t **out) {
@@ -1079,16 +1087,18 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream) {
   return CSADV_OK;
 }
 
-int csadv_finalize(csadv_t *h, void *stream) {
+int csadv_finalize(csadv_t *h, int32_t solve, void *stream) {
   if (!h || !h->dev) return CSADV_E_STATE;
   hipStream_t st = (hipStream_t)stream;
   const int ns = (int)h->desc.size();
-  hipLaunchKernelGGL(k_settle, dim3(ns), dim3(BLOCK), 0, st, h->dev);
+  if (solve < -1 || solve >= ns) return fail(h, CSADV_E_ARG, "finalize: bad solve index");
+  const int first = solve < 0 ? 0 : solve, count = solve < 0 ? ns : 1;
+  hipLaunchKernelGGL(k_settle, dim3(count), dim3(BLOCK), 0, st, h->dev, first);
   if (h->dtype == CSADV_F64)
-    hipLaunchKernelGGL(k_finalize<double>, dim3(256, ns), dim3(BLOCK), 0, st, h->dev);
+    hipLaunchKernelGGL(k_finalize<double>, dim3(256, count), dim3(BLOCK), 0, st, h->dev, first);
   else
-    hipLaunchKernelGGL(k_finalize<float>, dim3(256, ns), dim3(BLOCK), 0, st, h->dev);
-  hipLaunchKernelGGL(k_clear_corr, dim3(ns), dim3(64), 0, st, h->dev);
+    hipLaunchKernelGGL(k_finalize<float>, dim3(256, count), dim3(BLOCK), 0, st, h->dev, first);
+  hipLaunchKernelGGL(k_clear_corr, dim3(count), dim3(64), 0, st, h->dev, first);
   HIPCHK(h, hipGetLastError());
   return CSADV_OK;
 }
@@ -1098,7 +1108,7 @@ int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *str
   if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr || !out)
     return fail(h, CSADV_E_ARG, "mass: bad solve / tracer / output");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_settle, dim3((int)h->desc.size()), dim3(BLOCK), 0, st, h->dev);
+  hipLaunchKernelGGL(k_settle, dim3(1), dim3(BLOCK), 0, st, h->dev, solve);
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_mass<double>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, h->red);
   else
@@ -1118,7 +1128,7 @@ int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact,
   if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr || !out || !qexact)
     return fail(h, CSADV_E_ARG, "norms: bad solve / tracer / pointer");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_settle, dim3((int)h->desc.size()), dim3(BLOCK), 0, st, h->dev);
+  hipLaunchKernelGGL(k_settle, dim3(1), dim3(BLOCK), 0, st, h->dev, solve);
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_norms<double>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, qexact, h->red);
   else

@@ -10,11 +10,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcsadv.so")
 
 CSADV_F64, CSADV_F32 = 0, 1
+ABI_VERSION = 2          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
 _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of order", -4: "size out of range"}
 
 # exported symbols, in declaration order of include/csadv.h
 SYMBOLS = (
-    "csadv_create", "csadv_partials_per_tracer", "csadv_update", "csadv_wind_init",
+    "csadv_create", "csadv_partials_per_tracer", "csadv_tile_rows", "csadv_update", "csadv_wind_init",
     "csadv_ghost_fill", "csadv_step", "csadv_set_graph", "csadv_finalize", "csadv_mass", "csadv_norms",
     "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_last_error", "csadv_destroy",
     "csadv_abi_version",
@@ -41,6 +42,7 @@ class SolveDesc(ctypes.Structure):
         ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p), ("strip_gw", _p),
         ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p), ("corner_gw", _p),
         ("coef", _p), ("corr", _p), ("partials", _p), ("ctl", _p),
+        ("tile", ctypes.c_int32), ("pad_", ctypes.c_int32),
     ]
 
 
@@ -62,12 +64,13 @@ def load():
     vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)
     lib.csadv_create.argtypes = [ctypes.POINTER(SolveDesc), i32, i32, ctypes.POINTER(vp)]
     lib.csadv_partials_per_tracer.argtypes = [i32, i32]
+    lib.csadv_tile_rows.argtypes = [i32]
     lib.csadv_update.argtypes = [vp, ctypes.POINTER(SolveDesc), i32, vp]
     lib.csadv_wind_init.argtypes = [vp, vp]
     lib.csadv_ghost_fill.argtypes = [vp, vp]
     lib.csadv_step.argtypes = [vp, i32, vp]
     lib.csadv_set_graph.argtypes = [vp, i32]
-    lib.csadv_finalize.argtypes = [vp, vp]
+    lib.csadv_finalize.argtypes = [vp, i32, vp]
     lib.csadv_mass.argtypes = [vp, i32, i32, dbl, vp]
     lib.csadv_norms.argtypes = [vp, i32, i32, vp, dbl, vp]
     lib.csadv_current.argtypes = [vp, i32]
@@ -81,7 +84,7 @@ def load():
     for name in SYMBOLS:
         if name not in ("csadv_last_error", "csadv_destroy"):
             getattr(lib, name).restype = ctypes.c_int
-    if lib.csadv_abi_version() != 1:
+    if lib.csadv_abi_version() != ABI_VERSION:
         raise CsadvError("libcsadv ABI version mismatch")
     _lib = lib
     return lib

@@ -1,0 +1,107 @@
+"""Batched form: many independent solves advanced by one handle.
+
+Every kernel of a step is one launch over all solves of the batch (uneven N and tracer
+counts share the launch; each solve keeps its own buffers, tables and tile shape), and
+groups of steps are replayed from one captured HIP graph.  A solve's arithmetic does not
+depend on its companions: its tile is its own, its mass-fixer sums are its own and reading
+one result folds only that solve's pending correction, so a solve gives the same bits
+alone and in any batch.
+
+The reference runs one solve per process (src/advection_sphere.py:13-61); this module has
+no counterpart there beyond running ``adv_time_step`` for each member.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+from .solver import Advection, cfl_dt, grid_tables
+
+
+class Batch:
+    """One handle over ``members`` (constructed :class:`Advection` solves, not initialised).
+
+    All members share dtype, recon and dp (one kernel instantiation per launch)."""
+
+    def __init__(self, members, graph_steps=16):
+        if not members:
+            raise ValueError("empty batch")
+        first = members[0]
+        for m in members:
+            if (m.dtype, m.opts["recon"], m.opts["dp"]) != (first.dtype, first.opts["recon"], first.opts["dp"]):
+                raise ValueError("a batch shares dtype, recon and dp")
+            if m.handle is not None:
+                raise ValueError("batch members must not be initialised on their own")
+        self.members = list(members)
+        self.lib = first.lib
+        self.device = first.device
+        self.handle = None
+        self.graph_steps = int(graph_steps)
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def init(self):
+        """Initial states of every member and one handle over all of them."""
+        for m in self.members:
+            m.init_buffers()
+        descs = (_lib.SolveDesc * len(self.members))(*[m.desc() for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self.lib.csadv_create(descs, len(self.members), 0, ctypes.byref(h))
+        self.handle = h
+        _lib.check(rc, h, "csadv_create")
+        for i, m in enumerate(self.members):
+            m.handle, m.index, m.owner = h, i, self
+        _lib.check(self.lib.csadv_wind_init(h, self._stream()), h, "csadv_wind_init")
+        if self.graph_steps > 0:
+            _lib.check(self.lib.csadv_set_graph(h, self.graph_steps), h, "csadv_set_graph")
+        return self
+
+    def step(self, nsteps=1):
+        for m in self.members:
+            if not m.case.steady and m.steps_done + nsteps > m.max_steps:
+                raise _lib.CsadvError("unsteady winds: coefficient table holds %d steps" % m.max_steps)
+        _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
+        for m in self.members:
+            m.steps_done += nsteps
+
+    def cell_updates_per_step(self):
+        return sum(6 * m.grid.N ** 2 * m.case.ntr for m in self.members)
+
+    def profile(self, capacity):
+        _lib.check(self.lib.csadv_profile(self.handle, int(capacity)), self.handle, "csadv_profile")
+
+    def profile_read(self):
+        ms, n = ctypes.c_double(), ctypes.c_int32()
+        _lib.check(self.lib.csadv_profile_read(self.handle, ctypes.byref(ms), ctypes.byref(n)), self.handle,
+                   "csadv_profile_read")
+        return ms.value, n.value
+
+    def use_graph(self, steps_per_graph):
+        _lib.check(self.lib.csadv_set_graph(self.handle, int(steps_per_graph)), self.handle, "csadv_set_graph")
+
+    def close(self):
+        if self.handle:
+            for m in self.members:
+                m.handle, m.owner = None, None
+            self.lib.csadv_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_members(specs, dtype="f32", cfl=0.5, device="cuda:0", tiles=None, **opts):
+    """Advection solves of ``specs`` = [(N, case, nsteps)], dt from the CFL number at t=0,
+    tile per N from ``tiles`` ({N: tile}; default 0), tables shared per (N, dtype)."""
+    out = []
+    for N, case, nsteps in specs:
+        tab = grid_tables(N, dtype, device)
+        dt = cfl_dt(tab.grid, case, cfl)
+        tile = (tiles or {}).get(N, 0)
+        out.append(Advection(N, case, dt, dtype=dtype, device=device, tile=tile, max_steps=max(1, nsteps),
+                             tables=tab, **opts))
+    return out

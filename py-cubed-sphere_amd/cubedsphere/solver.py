@@ -76,6 +76,17 @@ class GridTables:
         self.conv_pv = up(_pad_last(np.stack([pv.ex_lon, pv.ex_lat, pv.ey_lon, pv.ey_lat, pv.det]), ld))
 
 
+_TABLES = {}
+
+
+def grid_tables(N, dtype, device):
+    """GridTables of (N, dtype, device), built once per process and shared by every solve."""
+    key = (int(N), dtype, str(torch.device(device)))
+    if key not in _TABLES:
+        _TABLES[key] = GridTables(Grid(N), dtype, torch.device(device))
+    return _TABLES[key]
+
+
 def interior_winds(grid, case):
     """Contravariant basis winds at the interior edges (src/advection_vars.py:37-53):
     returns (wu [K][6][M][M+1], wv [K][6][M+1][M]) with zeros outside the interior."""
@@ -129,6 +140,8 @@ class Advection:
         self.tab = tables if tables is not None else GridTables(self.grid, dtype, self.device)
         self.max_steps = max(1, int(max_steps))
         self.handle = None
+        self.index = 0              # solve index inside the handle (a batch shares one handle)
+        self.owner = None           # the Batch whose handle this solve uses, if any
         self.steps_done = 0
         self._alloc()
 
@@ -182,6 +195,7 @@ class Advection:
         d.corner_tgt, d.corner_src, d.corner_w, d.corner_gw = (_ptr(tab.corner_tgt), _ptr(tab.corner_src),
                                                                _ptr(tab.corner_w), _ptr(tab.corner_gw))
         d.coef, d.corr, d.partials, d.ctl = _ptr(self.coef), _ptr(self.corr), _ptr(self.partials), _ptr(self.ctl)
+        d.tile = self.tile
         return d
 
     def _stream(self):
@@ -189,6 +203,19 @@ class Advection:
 
     # ------------------------------------------------------------ init (src/advection_vars.py:19-107)
     def init_vars_adv(self):
+        """Initial state, wind basis and a handle of this one solve."""
+        self.init_buffers()
+        h = ctypes.c_void_p()
+        d = self.desc()
+        rc = self.lib.csadv_create(ctypes.byref(d), 1, self.tile, ctypes.byref(h))
+        self.handle = h
+        _lib.check(rc, h, "csadv_create")
+        _lib.check(self.lib.csadv_wind_init(h, self._stream()), h, "csadv_wind_init")
+        return self
+
+    def init_buffers(self):
+        """Initial condition at the interior, zero ghosts, wind-basis interiors, cleared
+        control words (the device part of init_vars_adv; no handle yet)."""
         g = self.grid
         i0, iend = g.i0, g.iend
         wu, wv = interior_winds(g, self.case)
@@ -203,12 +230,6 @@ class Advection:
         self.ctl.zero_()
         self.corr.zero_()
         torch.cuda.synchronize(self.device)
-        h = ctypes.c_void_p()
-        d = self.desc()
-        rc = self.lib.csadv_create(ctypes.byref(d), 1, self.tile, ctypes.byref(h))
-        self.handle = h
-        _lib.check(rc, h, "csadv_create")
-        _lib.check(self.lib.csadv_wind_init(h, self._stream()), h, "csadv_wind_init")
         self.steps_done = 0
         return self
 
@@ -217,6 +238,8 @@ class Advection:
         """nsteps calls of adv_time_step + update_adv (src/advection_timestep.py:19-75)."""
         if self.handle is None:
             raise _lib.CsadvError("init_vars_adv() first")
+        if self.owner is not None:
+            raise _lib.CsadvError("this solve is stepped by its batch (Batch.step)")
         if not self.case.steady and self.steps_done + nsteps > self.max_steps:
             raise _lib.CsadvError("unsteady winds: coefficient table holds %d steps" % self.max_steps)
         _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
@@ -236,10 +259,10 @@ class Advection:
         return self.uavg[:, :, :M + 1].double().cpu().numpy(), self.vavg[:, :, :M].double().cpu().numpy()
 
     def current(self):
-        return _lib.check(self.lib.csadv_current(self.handle, 0), self.handle, "csadv_current")
+        return _lib.check(self.lib.csadv_current(self.handle, self.index), self.handle, "csadv_current")
 
     def finalize(self):
-        _lib.check(self.lib.csadv_finalize(self.handle, self._stream()), self.handle, "csadv_finalize")
+        _lib.check(self.lib.csadv_finalize(self.handle, self.index, self._stream()), self.handle, "csadv_finalize")
 
     def state(self):
         """Interior of every tracer after folding in pending corrections: [ntr][6][N][N] FP64 numpy."""
@@ -256,7 +279,8 @@ class Advection:
 
     def mass(self, tracer=0):
         out = ctypes.c_double()
-        _lib.check(self.lib.csadv_mass(self.handle, 0, tracer, ctypes.byref(out), self._stream()), self.handle, "csadv_mass")
+        _lib.check(self.lib.csadv_mass(self.handle, self.index, tracer, ctypes.byref(out), self._stream()), self.handle,
+                   "csadv_mass")
         return out.value
 
     def exact(self, tracer=0, t=None):
@@ -269,7 +293,8 @@ class Advection:
         """(Linf, L1, L2) against the exact solution (src/errors.py:99-113)."""
         qe = torch.from_numpy(np.ascontiguousarray(self.exact(tracer, t))).to(self.device, torch.float64)
         out = (ctypes.c_double * 3)()
-        _lib.check(self.lib.csadv_norms(self.handle, 0, tracer, _ptr(qe), out, self._stream()), self.handle, "csadv_norms")
+        _lib.check(self.lib.csadv_norms(self.handle, self.index, tracer, _ptr(qe), out, self._stream()), self.handle,
+                   "csadv_norms")
         return tuple(out)
 
     def profile(self, capacity):
@@ -284,9 +309,9 @@ class Advection:
         return ms.value, n.value
 
     def close(self):
-        if self.handle:
+        if self.handle and self.owner is None:
             self.lib.csadv_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -1,0 +1,97 @@
+"""Batched form on the device: one handle over solves of uneven N and tracer count.
+
+Bar: a solve gives the same bits alone and in any batch (its arithmetic does not depend on
+its companions); FP64 batch members match the oracle within 1e-12 relative; FP32 members
+stay within 1e-5 relative L2 of FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import interior_dev, rel_max, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device visible")
+
+
+def _specs(nsteps):
+    from cubedsphere.cases import random_solid_body, reference_case, workload
+    rng = np.random.default_rng(7)
+    return [(16, reference_case(2, 1), nsteps),
+            (24, workload("nl10_deformation"), nsteps),          # unsteady, 4 tracers
+            (20, random_solid_body(rng, 3), nsteps),
+            (32, random_solid_body(rng, 1), nsteps)]
+
+
+def test_batch_members_equal_solves_run_alone():
+    from cubedsphere.batch import Batch, make_members
+    nsteps = 12
+    specs = _specs(nsteps)
+    tiles = {16: 1, 24: 0, 20: 7, 32: 2}
+    batch = Batch(make_members(specs, dtype="f64", tiles=tiles), graph_steps=4).init()
+    batch.step(5)
+    batch.step(7)
+    for i, spec in enumerate(specs):
+        alone = Batch(make_members([spec], dtype="f64", tiles=tiles), graph_steps=0).init()
+        alone.step(nsteps)
+        a, b = batch.members[i].full_state(), alone.members[0].full_state()
+        assert np.array_equal(a, b), i
+        alone.close()
+
+
+def test_batch_member_matches_oracle():
+    from cubedsphere.batch import Batch, make_members
+    nsteps = 6
+    specs = _specs(nsteps)
+    batch = Batch(make_members(specs, dtype="f64"), graph_steps=0).init()
+    batch.step(nsteps)
+    for i in (0, 1):
+        m = batch.members[i]
+        grid, sim = run_oracle(m.grid.N, m.case, m.dt, nsteps)
+        got = m.state()
+        refs = sim.Qs if hasattr(sim, "Qs") else [sim.Q]
+        for n, ref in enumerate(refs):
+            assert rel_max(got[n], interior_dev(grid, ref)) < 1e-12, (i, n)
+
+
+def test_fp32_batch_within_l2_of_fp64():
+    from cubedsphere.batch import Batch, make_members
+    nsteps = 20
+    specs = _specs(nsteps)
+    b32 = Batch(make_members(specs, dtype="f32"), graph_steps=8).init()
+    b64 = Batch(make_members(specs, dtype="f64"), graph_steps=8).init()
+    b32.step(nsteps)
+    b64.step(nsteps)
+    for m32, m64 in zip(b32.members, b64.members):
+        a, b = m32.state(), m64.state()
+        for n in range(a.shape[0]):
+            l2 = np.linalg.norm(a[n] - b[n]) / np.linalg.norm(b[n])
+            assert l2 <= 1e-5, (m32.grid.N, n, l2)
+
+
+def test_finalize_of_one_member_leaves_the_others_untouched():
+    """Reading one result mid-run must not change the bits of the solves still running."""
+    from cubedsphere.batch import Batch, make_members
+    specs = _specs(10)
+    x = Batch(make_members(specs, dtype="f64"), graph_steps=0).init()
+    y = Batch(make_members(specs, dtype="f64"), graph_steps=0).init()
+    x.step(5)
+    y.step(5)
+    x.members[0].state()                    # fold member 0's correction in x only
+    x.step(5)
+    y.step(5)
+    for i in range(1, len(specs)):
+        assert np.array_equal(x.members[i].full_state(), y.members[i].full_state()), i
+
+
+def test_autotuner_returns_a_valid_tile():
+    from cubedsphere.autotune import tile_ids, tune_solve
+    best, timings = tune_solve(48, dtype="f32", ntr=2, candidates=[0, 1, 2], steps=4, warmup=2)
+    assert best in (0, 1, 2) and set(timings) == {0, 1, 2}
+    assert all(t > 0 for t in timings.values())
+    assert len(tile_ids()) >= 6

@@ -178,7 +178,7 @@ def test_graph_replay_bitwise_equal_plain_launches():
     assert rel_max(graph.state()[0], interior_dev(grid, sim.Q)) < TOL64
 
 
-@pytest.mark.parametrize("tile", range(6))
+@pytest.mark.parametrize("tile", range(12))
 def test_every_tile_shape_matches_oracle(tile):
     N, dt, nsteps = 40, 0.008, 6
     case = _ref(2, 1)

@@ -180,9 +180,13 @@ def test_library_loads_and_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert set(declared) == set(_lib.SYMBOLS)
     loaded = _lib.load()
-    assert loaded.csadv_abi_version() == 1
+    assert loaded.csadv_abi_version() == 2
     assert loaded.csadv_partials_per_tracer(384, 0) > 0
     assert loaded.csadv_partials_per_tracer(384, 99) < 0
+    rows = []
+    while loaded.csadv_tile_rows(len(rows)) > 0:
+        rows.append(loaded.csadv_tile_rows(len(rows)))
+    assert len(rows) >= 6 and len(set(rows)) == len(rows)
 
 
 def test_descriptor_struct_matches_header_field_order():
