@@ -49,7 +49,9 @@ WORKLOADS = {
     "nl10_n768_f32": ("single", 768, "nl10_deformation", "f32", 0.5, 300),
     "batched_64_f32": ("batched", None, None, "f32", 0.5, 200),
     "polar_n48_f64": ("single", 48, "solid_body_polar", "f64", 0.5, 384),
+    "served_poisson_f32": ("served", None, None, "f32", 0.5, None),
 }
+SERVED_REQUESTS, SERVED_RATE = 200, 12.0      # Poisson stream: count, arrivals per second
 BATCH_NS = (32, 48, 96, 192, 384, 512)
 # tile ids from the autotuner for the single-solve workloads (the batched workload tunes
 # itself at start-up over BATCH_TILE_CANDIDATES)
@@ -179,6 +181,9 @@ def main():
     from cubedsphere.cases import workload
 
     kind, N, case_name, dtype, cfl, default_steps = WORKLOADS[args.workload]
+    if kind == "served":
+        print(json.dumps(run_served(args)))
+        return
     K = args.steps if args.steps is not None else default_steps
     W = args.warmup
     nstep_total = W + 2 * K + max(args.graph, 0)
@@ -191,10 +196,13 @@ def main():
         if args.tile is not None:
             tile = args.tile
         else:
-            tile, timings = tune_batch(specs, dtype=dtype, candidates=BATCH_TILE_CANDIDATES, steps=16, warmup=8)
+            tile, timings = tune_batch(specs, dtype=dtype, candidates=BATCH_TILE_CANDIDATES, steps=16, warmup=8,
+                                       device_grid=True)
             tuned = {str(t): round(v * 1e6, 1) for t, v in timings.items()}
     tiles = {s[0]: tile for s in specs}
-    members = make_members(specs, dtype=dtype, cfl=cfl, tiles=tiles)
+    # many solves: geometry, fields and winds prepared on the device
+    device_grid = kind == "batched"
+    members = make_members(specs, dtype=dtype, cfl=cfl, tiles=tiles, device_grid=device_grid)
     stepper = Batch(members, graph_steps=0).init()
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * (1 << 20) // 8
@@ -250,7 +258,8 @@ def main():
     if dtype == "f32" and not args.no_l2:
         # the same solves in FP64 for as many steps as the FP32 run took
         done = members[0].steps_done
-        m64 = make_members([(s[0], s[1], done) for s in specs], dtype="f64", cfl=cfl, tiles=tiles)
+        m64 = make_members([(s[0], s[1], done) for s in specs], dtype="f64", cfl=cfl, tiles=tiles,
+                           device_grid=device_grid)
         b64 = Batch(m64, graph_steps=16).init()
         b64.step(done)
         out["l2_vs_f64"] = l2_vs_f64(members, m64)
@@ -260,6 +269,41 @@ def main():
         m0 = members[0]
         out["cpu_baseline"] = cpu_baseline(m0.grid.N, m0.case, m0.dt, args.cpu_seconds)
     print(json.dumps(out))
+
+
+def run_served(args):
+    """Served form: a seeded Poisson stream of requests through the continuous-batching
+    server (N log-uniform in [48, 1024], 1-4 tracers, 50-2000 steps, Gaussian or cap
+    fields, tilted solid-body or deformational wind, FP32 with 10% FP64 requests)."""
+    from cubedsphere.serve import Server, random_requests
+    reqs = random_requests(args.seed, SERVED_REQUESTS, SERVED_RATE)
+    Server(chunk=16).run(random_requests(args.seed + 1, 4, 1000.0, steps=(20, 40)))   # warm-up
+    srv = Server(chunk=16, budget_bytes=96 << 30, max_slots=64)
+    srv.run(reqs)
+    st = srv.stats
+    return {
+        "metric": "tracer cell-updates/s",
+        "value": st["cell_updates_per_s"],
+        "unit": "tracer cell-updates/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 4,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Poisson request stream of analytic cases)",
+        "config": {"workload": args.workload, "requests": len(reqs), "rate_per_s": SERVED_RATE,
+                   "chunk_steps": 16, "budget_bytes": 96 << 30, "max_slots": 64,
+                   "f64_requests": sum(r.dtype == "f64" for r in reqs)},
+        "p50_latency_s": st["p50_latency_s"],
+        "p99_latency_s": st["p99_latency_s"],
+        "served": st,
+        "roofline": None,
+        "roofline_note": "served rounds replay graphs; the step kernel's roofline is measured by the "
+                         "single-solve and batched workloads",
+    }
 
 
 if __name__ == "__main__":

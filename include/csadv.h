@@ -87,7 +87,10 @@ typedef struct csadv_solve_desc {
                                       -1 = the handle's tile.  The per-wavefront mass-fixer
                                       partial sums follow the tile, so a solve keeps the same
                                       bits alone and in any batch when its tile is fixed here */
-  int32_t pad_;
+  int32_t stop;                    /* > 0: the step kernels leave this solve untouched once it
+                                      has completed `stop` steps (a batch can then run a fixed
+                                      number of steps while its solves end at their own step);
+                                      0 = no limit                                           */
 } csadv_solve_desc;
 
 typedef struct csadv_s csadv_t;

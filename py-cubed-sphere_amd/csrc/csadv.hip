@@ -155,6 +155,10 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
   const int ncell = D.n_strip + D.n_corner;
   if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
   const int steps = D.ctl[1];
+  if (D.stop > 0 && steps >= D.stop) {      // solve finished: k_step reads ctl[2] and skips it too
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
+    return;
+  }
   T *Q = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0);
   const int idx = blockIdx.x * BLOCK + threadIdx.x;
   const bool active = idx < ncell;
@@ -378,6 +382,7 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   const int iend = N + NG;
   const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
   const int step = D.ctl[2];
+  if (D.stop > 0 && step >= D.stop) return;   // solve finished
   const int item = (p * S.segs + wk.w) * S.strips + wk.z;
   const bool mfpr = D.mf == 3;
   if (item == 0 && lane == 0) {      // hand the step over to the next kernels of the stream
@@ -566,6 +571,7 @@ __global__ void k_wind_avg(Solve *solves) {
   const int N = D.N, M = S.M, ld = D.ld, ldu = D.ldu, K = D.nbasis;
   const int p = blockIdx.y;
   const int i0 = NG, iend = N + NG;
+  if (D.stop > 0 && D.ctl[1] >= D.stop) return;                    // solve finished
   const int row = D.unsteady ? min(D.ctl[1], D.max_steps - 1) : 0;   // steps completed
   const double *s1 = D.coef + (int64_t)row * 2 * K;   // coefficients at t_{n-1}
   const double *s0 = s1 + K;                          // at t_{n-2}
@@ -696,7 +702,7 @@ struct csadv_s {
   int recon = 3;
   bool rk2 = false;               // dp = 2 (separate upwind-side winds)
   double *red = nullptr;          // small reduction scratch (mass / norms)
-  std::vector<int> cur;           // host copy of the current-buffer index per solve
+  std::vector<int> done;          // host copy of the steps completed per solve (ctl[1])
   std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
   int ev_used = 0;
   int graph_steps = 0;            // steps per captured graph (0 = plain launches)
@@ -764,6 +770,7 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "strip_src and corner_src must be 16-byte aligned";
     return CSADV_E_ARG;
   }
+  if (d.stop < 0) { msg = "stop must be >= 0"; return CSADV_E_RANGE; }
   if (d.tile < -1 || d.tile >= kNumTiles) { msg = "tile must be -1 or a valid tile id"; return CSADV_E_RANGE; }
   if (!(d.dt > 0.0) || !(d.dx > 0.0) || !(d.dy > 0.0)) { msg = "dt, dx, dy must be positive"; return CSADV_E_ARG; }
   return CSADV_OK;
@@ -893,7 +900,7 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
     }
     h->desc.push_back(solves[s]);
   }
-  h->cur.assign(nsolve, 0);
+  h->done.assign(nsolve, 0);
   *out = h;
   hipError_t e;
   if ((e = hipMalloc(&h->dev, sizeof(Solve) * nsolve)) != hipSuccess ||
@@ -957,7 +964,12 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
   HIPCHK(h, hipMemcpyAsync(h->work, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice, st));
   HIPCHK(h, hipMemcpyAsync(h->dev, h->host.data(), sizeof(Solve) * nsolve, hipMemcpyHostToDevice, st));
   HIPCHK(h, hipMemcpyAsync(h->qsets, sets.data(), sizeof(FieldSet) * nsolve, hipMemcpyHostToDevice, st));
+  // steps already completed by each solve (a handle may be built over solves that have run)
+  std::vector<int32_t> ctl(4 * nsolve);
+  for (int s = 0; s < nsolve; ++s)
+    HIPCHK(h, hipMemcpyAsync(ctl.data() + 4 * s, solves[s].ctl, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
+  for (int s = 0; s < nsolve; ++s) h->done[s] = ctl[4 * s + 1];
   return CSADV_OK;
 }
 
@@ -1083,7 +1095,11 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream) {
     const int rc = enqueue_step(h, st, true);
     if (rc) return rc;
   }
-  for (int s = 0; s < ns; ++s) h->cur[s] = (h->cur[s] + nsteps) & 1;
+  for (int s = 0; s < ns; ++s) {
+    const int stop = h->desc[s].stop;
+    h->done[s] += nsteps;
+    if (stop > 0 && h->done[s] > stop) h->done[s] = std::max(stop, h->done[s] - nsteps);
+  }
   return CSADV_OK;
 }
 
@@ -1152,8 +1168,8 @@ int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact,
 }
 
 int csadv_current(csadv_t *h, int32_t solve) {
-  if (!h || solve < 0 || solve >= (int)h->cur.size()) return CSADV_E_ARG;
-  return h->cur[solve];
+  if (!h || solve < 0 || solve >= (int)h->done.size()) return CSADV_E_ARG;
+  return h->done[solve] & 1;
 }
 
 int csadv_profile(csadv_t *h, int32_t capacity) {

@@ -42,7 +42,7 @@ class SolveDesc(ctypes.Structure):
         ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p), ("strip_gw", _p),
         ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p), ("corner_gw", _p),
         ("coef", _p), ("corr", _p), ("partials", _p), ("ctl", _p),
-        ("tile", ctypes.c_int32), ("pad_", ctypes.c_int32),
+        ("tile", ctypes.c_int32), ("stop", ctypes.c_int32),
     ]
 
 

@@ -15,7 +15,7 @@ import ctypes
 import torch
 
 from . import _lib
-from .solver import Advection, cfl_dt, grid_tables
+from .solver import Advection, cfl_dt, grid_tables, interior_winds
 
 
 class Batch:
@@ -45,25 +45,54 @@ class Batch:
         """Initial states of every member and one handle over all of them."""
         for m in self.members:
             m.init_buffers()
-        descs = (_lib.SolveDesc * len(self.members))(*[m.desc() for m in self.members])
+        self._open(self.members)
+        return self
+
+    def _create(self, members):
+        descs = (_lib.SolveDesc * len(members))(*[m.desc() for m in members])
         h = ctypes.c_void_p()
-        rc = self.lib.csadv_create(descs, len(self.members), 0, ctypes.byref(h))
+        rc = self.lib.csadv_create(descs, len(members), 0, ctypes.byref(h))
+        if rc != 0:
+            msg = self.lib.csadv_last_error(h).decode()
+            self.lib.csadv_destroy(h)
+            raise _lib.CsadvError("csadv_create: %s" % msg)
+        return h
+
+    def _open(self, fresh):
+        """Prepare the winds of the fresh members (on a handle of their own, so members that
+        are already running are not touched), then one handle over every member."""
+        if fresh:
+            h = self._create(fresh)
+            rc = self.lib.csadv_wind_init(h, self._stream())
+            msg = self.lib.csadv_last_error(h).decode() if rc else ""
+            self.lib.csadv_destroy(h)
+            if rc:
+                raise _lib.CsadvError("csadv_wind_init: %s" % msg)
+            for m in fresh:
+                m.release_wind_scratch()
+        h = self._create(self.members)
         self.handle = h
-        _lib.check(rc, h, "csadv_create")
         for i, m in enumerate(self.members):
             m.handle, m.index, m.owner = h, i, self
-        _lib.check(self.lib.csadv_wind_init(h, self._stream()), h, "csadv_wind_init")
         if self.graph_steps > 0:
             _lib.check(self.lib.csadv_set_graph(h, self.graph_steps), h, "csadv_set_graph")
-        return self
+
+    def rebuild(self, members, fresh=()):
+        """Continue with ``members`` (solves already in the batch keep their state; ``fresh``
+        ones must have had init_buffers() and get their winds prepared here)."""
+        self._close_handle()
+        self.members = list(members)
+        if self.members:
+            self._open(list(fresh))
 
     def step(self, nsteps=1):
         for m in self.members:
-            if not m.case.steady and m.steps_done + nsteps > m.max_steps:
+            todo = nsteps if not m.stop else max(0, min(nsteps, m.stop - m.steps_done))
+            if not m.case.steady and m.steps_done + todo > m.max_steps:
                 raise _lib.CsadvError("unsteady winds: coefficient table holds %d steps" % m.max_steps)
         _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
         for m in self.members:
-            m.steps_done += nsteps
+            m.steps_done += nsteps if not m.stop else max(0, min(nsteps, m.stop - m.steps_done))
 
     def cell_updates_per_step(self):
         return sum(6 * m.grid.N ** 2 * m.case.ntr for m in self.members)
@@ -80,12 +109,15 @@ class Batch:
     def use_graph(self, steps_per_graph):
         _lib.check(self.lib.csadv_set_graph(self.handle, int(steps_per_graph)), self.handle, "csadv_set_graph")
 
-    def close(self):
+    def _close_handle(self):
         if self.handle:
             for m in self.members:
                 m.handle, m.owner = None, None
             self.lib.csadv_destroy(self.handle)
             self.handle = None
+
+    def close(self):
+        self._close_handle()
 
     def __del__(self):
         try:
@@ -94,14 +126,19 @@ class Batch:
             pass
 
 
-def make_members(specs, dtype="f32", cfl=0.5, device="cuda:0", tiles=None, **opts):
-    """Advection solves of ``specs`` = [(N, case, nsteps)], dt from the CFL number at t=0,
-    tile per N from ``tiles`` ({N: tile}; default 0), tables shared per (N, dtype)."""
-    out = []
-    for N, case, nsteps in specs:
-        tab = grid_tables(N, dtype, device)
-        dt = cfl_dt(tab.grid, case, cfl)
-        tile = (tiles or {}).get(N, 0)
-        out.append(Advection(N, case, dt, dtype=dtype, device=device, tile=tile, max_steps=max(1, nsteps),
-                             tables=tab, **opts))
-    return out
+def make_member(N, case, nsteps, dtype="f32", cfl=0.5, device="cuda:0", tile=0, device_grid=False, **opts):
+    """One Advection solve with dt from the CFL number at t=0 (tables shared per N and dtype).
+    device_grid: geometry, initial fields and winds evaluated on the device (FP64 torch)."""
+    tab = grid_tables(N, dtype, device, device_grid=device_grid)
+    winds = interior_winds(tab.grid, case)
+    dt = cfl_dt(tab.grid, case, cfl, winds=winds)
+    m = Advection(N, case, dt, dtype=dtype, device=device, tile=tile, max_steps=max(1, nsteps), tables=tab, **opts)
+    m.basis_winds = winds
+    return m
+
+
+def make_members(specs, dtype="f32", cfl=0.5, device="cuda:0", tiles=None, device_grid=False, **opts):
+    """Advection solves of ``specs`` = [(N, case, nsteps)], tile per N from ``tiles``
+    ({N: tile}; default 0)."""
+    return [make_member(N, case, nsteps, dtype=dtype, cfl=cfl, device=device, tile=(tiles or {}).get(N, 0),
+                        device_grid=device_grid, **opts) for N, case, nsteps in specs]

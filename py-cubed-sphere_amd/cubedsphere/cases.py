@@ -17,51 +17,72 @@ fields) follow Williamson et al. (1992) test 1 and Nair & Lauritzen (2010) sec. 
 import math
 
 import numpy as np
+import torch
 
 PI = math.pi
 DEG2RAD = 1.0 / (180.0 / PI)
 
 
+# Every field and wind below takes numpy arrays (host, the parity path) or torch tensors
+# (device preparation of the batched and served forms) and computes with the same library.
+def _xp(a):
+    return torch if torch.is_tensor(a) else np
+
+
+def sin(a):
+    return _xp(a).sin(a)
+
+
+def cos(a):
+    return _xp(a).cos(a)
+
+
 def sph2cart(lon, lat):
-    return np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)
+    return cos(lat) * cos(lon), cos(lat) * sin(lon), sin(lat)
 
 
 # ---------------------------------------------------------------- scalar fields
 def gaussian_sum(X, Y, Z, centres, b, hmax=1.0):
-    q = np.zeros(np.shape(X))
+    xp = _xp(X)
+    q = xp.zeros_like(X)
     for lon0, lat0 in centres:
         X0, Y0, Z0 = sph2cart(lon0, lat0)
-        q = q + hmax * np.exp(-b * ((X - X0) ** 2 + (Y - Y0) ** 2 + (Z - Z0) ** 2))
+        q = q + hmax * xp.exp(-b * ((X - X0) ** 2 + (Y - Y0) ** 2 + (Z - Z0) ** 2))
     return q
 
 
 def _gc_dist(lon, lat, lon0, lat0):
-    c = np.sin(lat0) * np.sin(lat) + np.cos(lat0) * np.cos(lat) * np.cos(lon - lon0)
-    return np.arccos(np.clip(c, -1.0, 1.0))
+    xp = _xp(lon)
+    c = math.sin(lat0) * sin(lat) + math.cos(lat0) * cos(lat) * cos(lon - lon0)
+    return xp.arccos(xp.clip(c, -1.0, 1.0))
 
 
 def cosine_bells(lon, lat, centres, radius=0.5, base=0.1, scale=0.9):
-    q = np.zeros(np.shape(lon))
+    xp = _xp(lon)
+    q = xp.zeros_like(lon)
     for lon0, lat0 in centres:
         r = _gc_dist(lon, lat, lon0, lat0)
-        q = q + np.where(r < radius, 0.5 * (1.0 + np.cos(PI * r / radius)), 0.0)
+        q = q + xp.where(r < radius, 0.5 * (1.0 + cos(PI * r / radius)), 0.0)
     return base + scale * q
 
 
 def slotted_cylinders(lon, lat, centres, radius=0.5, high=1.0, low=0.1):
-    q = np.full(np.shape(lon), low)
+    xp = _xp(lon)
+    q = xp.full_like(lon, low)
     for n, (lon0, lat0) in enumerate(centres):
         r = _gc_dist(lon, lat, lon0, lat0)
-        dlon = np.abs(np.angle(np.exp(1j * (lon - lon0))))
+        d = lon - lon0
+        dlon = xp.abs(xp.arctan2(sin(d), cos(d)))        # |lon - lon0| wrapped to [0, pi]
         slot = (lat - lat0 < -5.0 * radius / 12.0) if n % 2 == 0 else (lat - lat0 > 5.0 * radius / 12.0)
         keep = (dlon >= radius / 6.0) | ((dlon < radius / 6.0) & slot)
-        q = np.where((r <= radius) & keep, high, q)
+        q = xp.where((r <= radius) & keep, high, q)
     return q
 
 
 def named_field(spec, X, Y, Z):
-    lon = np.arctan2(Y, X)
-    lat = np.arctan2(Z, np.hypot(X, Y))
+    xp = _xp(X)
+    lon = xp.arctan2(Y, X)
+    lat = xp.arctan2(Z, xp.hypot(X, Y))
     kind = spec["field"]
     if kind == "gaussians":
         return gaussian_sum(X, Y, Z, spec["centres"], spec.get("b", 5.0), spec.get("hmax", 1.0))
@@ -78,13 +99,13 @@ def named_field(spec, X, Y, Z):
 def reference_q(ic, vf, lon, lat, t):
     """Exact solutions of src/advection_ic.py:215-281."""
     if ic == 1:
-        return np.ones(np.shape(lon))
+        return _xp(lon).ones_like(lon)
     X, Y, Z = sph2cart(lon, lat)
     if ic == 2:
         if vf == 1:
             alpha, u0 = -45.0 * DEG2RAD, 2.0 * PI / 5.0
             wt = -u0 * t
-            ca, sa, cw, sw = np.cos(alpha), np.sin(alpha), np.cos(wt), np.sin(wt)
+            ca, sa, cw, sw = math.cos(alpha), math.sin(alpha), math.cos(wt), math.sin(wt)
             rX = (cw * ca * ca + sa * sa) * X - sw * ca * Y + (cw * ca * sa - ca * sa) * Z
             rY = sw * ca * X + cw * Y + sa * sw * Z
             rZ = (cw * sa * ca - sa * ca) * X - sw * sa * Y + (cw * sa * sa + ca * ca) * Z
@@ -95,7 +116,7 @@ def reference_q(ic, vf, lon, lat, t):
         return gaussian_sum(X, Y, Z, cs, 5.0)
     if ic == 4:
         alpha = -45.0 * DEG2RAD
-        f = -np.cos(lon) * np.cos(lat) * np.sin(alpha) + np.sin(lat) * np.cos(alpha)
+        f = -cos(lon) * cos(lat) * math.sin(alpha) + sin(lat) * math.cos(alpha)
         return 1.0 - f * f
     raise ValueError("invalid initial condition %r" % ic)
 
@@ -145,33 +166,33 @@ class Case:
             vf = self.wind[1]
             if vf == 1:
                 alpha, u0 = -45.0 * DEG2RAD, 2.0 * PI / 5.0
-                return [(lambda lo, la: u0 * (np.cos(la) * np.cos(alpha) + np.sin(la) * np.cos(lo) * np.sin(alpha)),
-                         lambda lo, la: -u0 * np.sin(lo) * np.sin(alpha))]
+                return [(lambda lo, la: u0 * (cos(la) * cos(alpha) + sin(la) * cos(lo) * sin(alpha)),
+                         lambda lo, la: -u0 * sin(lo) * sin(alpha))]
             if vf == 2:
                 T, k = 5.0, 2.0
-                return [(lambda lo, la: 2.0 * PI * np.cos(la) / T, lambda lo, la: 0.0 * lo),
-                        (lambda lo, la: 0.5 * k * np.sin(2.0 * la), lambda lo, la: 0.0 * lo),
-                        (lambda lo, la: -0.5 * k * np.sin(2.0 * la) * np.cos(2.0 * lo),
-                         lambda lo, la: k * np.cos(la) * np.sin(2.0 * lo)),
-                        (lambda lo, la: -0.5 * k * np.sin(2.0 * la) * np.sin(2.0 * lo),
-                         lambda lo, la: -k * np.cos(la) * np.cos(2.0 * lo))]
+                return [(lambda lo, la: 2.0 * PI * cos(la) / T, lambda lo, la: 0.0 * lo),
+                        (lambda lo, la: 0.5 * k * sin(2.0 * la), lambda lo, la: 0.0 * lo),
+                        (lambda lo, la: -0.5 * k * sin(2.0 * la) * cos(2.0 * lo),
+                         lambda lo, la: k * cos(la) * sin(2.0 * lo)),
+                        (lambda lo, la: -0.5 * k * sin(2.0 * la) * sin(2.0 * lo),
+                         lambda lo, la: -k * cos(la) * cos(2.0 * lo))]
             if vf == 3:
                 k = 1.0
-                return [(lambda lo, la: -k * np.sin((lo + PI) / 2.0) ** 2 * np.sin(2.0 * la) * np.cos(la) ** 2,
-                         lambda lo, la: (k / 2.0) * np.sin(lo + PI) * np.cos(la) ** 3)]
+                return [(lambda lo, la: -k * sin((lo + PI) / 2.0) ** 2 * sin(2.0 * la) * cos(la) ** 2,
+                         lambda lo, la: (k / 2.0) * sin(lo + PI) * cos(la) ** 3)]
             if vf == 4:
-                return [(lambda lo, la: -(np.sin(lo) * np.sin(lo) * np.cos(la) ** 3),
-                         lambda lo, la: -4.0 * (np.cos(la) ** 3) * np.sin(la) * np.cos(lo) * np.sin(lo))]
+                return [(lambda lo, la: -(sin(lo) * sin(lo) * cos(la) ** 3),
+                         lambda lo, la: -4.0 * (cos(la) ** 3) * sin(la) * cos(lo) * sin(lo))]
             raise ValueError("invalid vector field %r" % vf)
         if kind == "solid_body":
             alpha, period = self.wind[1], self.wind[2]
             u0 = 2.0 * PI / period
-            return [(lambda lo, la: u0 * (np.cos(la) * np.cos(alpha) + np.sin(la) * np.cos(lo) * np.sin(alpha)),
-                     lambda lo, la: -u0 * np.sin(lo) * np.sin(alpha))]
+            return [(lambda lo, la: u0 * (cos(la) * cos(alpha) + sin(la) * cos(lo) * sin(alpha)),
+                     lambda lo, la: -u0 * sin(lo) * sin(alpha))]
         if kind == "deformational":
             k = self.wind[1]
-            return [(lambda lo, la: k * np.sin(lo / 2.0) ** 2 * np.sin(2.0 * la),
-                     lambda lo, la: (k / 2.0) * np.sin(lo) * np.cos(la))]
+            return [(lambda lo, la: k * sin(lo / 2.0) ** 2 * sin(2.0 * la),
+                     lambda lo, la: (k / 2.0) * sin(lo) * cos(la))]
         raise ValueError("unknown wind %r" % (kind,))
 
     def coefficients(self, t):

@@ -1,7 +1,9 @@
-"""FP64 geometry of the extended gnomonic cubed sphere, built on the host once per N.
+"""FP64 geometry of the extended gnomonic cubed sphere, built once per N.
 
-Everything here is init-time host work: the step itself runs in libcsadv.  Arrays are
-produced directly in the device layout ``[panel][j][i]`` (i = x fastest).
+Everything here is init-time work: the step itself runs in libcsadv.  Arrays are produced
+directly in the device layout ``[panel][j][i]`` (i = x fastest), as numpy arrays on the
+host (the parity path) or, with ``device=``, as FP64 torch tensors built on that device
+(the same formulas; used to prepare the many solves of the batched and served forms).
 
 Design: every panel is the panel-0 construction rotated by a signed permutation
 matrix (``PANEL_ROT``), so positions, tangent vectors and lat-lon unit vectors come
@@ -17,6 +19,7 @@ src/cs_datastruct.py:404-493) to rounding.
 import math
 
 import numpy as np
+import torch
 
 NG = 4
 PIO4 = math.pi / 4.0
@@ -43,14 +46,24 @@ def _rotate(p, c0, c1, c2):
 
 
 class PointSet:
-    """Geometry at one family of points (centres, x-edges or y-edges), layout [6][ny][nx]."""
+    """Geometry at one family of points (centres, x-edges or y-edges), layout [6][ny][nx].
 
-    def __init__(self, x, y):
+    device None: numpy on the host (the parity path, positions bitwise those of the
+    reference); a torch device: the same formulas in FP64 torch on that device (fast
+    preparation of many solves; agrees with the host path to rounding)."""
+
+    def __init__(self, x, y, device=None):
         # x varies along the last axis, y along the middle axis
-        yy, xx = np.meshgrid(y, x, indexing="ij")
-        tx, ty = np.tan(xx), np.tan(yy)
+        if device is None:
+            xp = np
+            yy, xx = np.meshgrid(y, x, indexing="ij")
+        else:
+            xp = torch
+            f64 = dict(dtype=torch.float64, device=device)
+            yy, xx = torch.meshgrid(torch.as_tensor(y, **f64), torch.as_tensor(x, **f64), indexing="ij")
+        tx, ty = xp.tan(xx), xp.tan(yy)
         D2 = 1.0 + tx**2 + ty**2
-        D = np.sqrt(D2)
+        D = xp.sqrt(D2)
         invD = 1.0 / D
         c0, c1, c2 = invD, invD * tx, invD * ty
         D3 = D2 * D
@@ -59,18 +72,36 @@ class PointSet:
         ex0 = (-tx * fx, (1.0 + ty * ty) * fx, -tx * ty * fx)
         ey0 = (-ty * fy, -tx * ty * fy, (1.0 + tx * tx) * fy)
         self.g = (1.0 + tx * tx) * (1.0 + ty * ty) / D3
-        shape = (6,) + xx.shape
-        self.X, self.Y, self.Z = np.empty(shape), np.empty(shape), np.empty(shape)
-        self.lon, self.lat = np.empty(shape), np.empty(shape)
-        self.ex_lon, self.ex_lat = np.empty(shape), np.empty(shape)
-        self.ey_lon, self.ey_lat = np.empty(shape), np.empty(shape)
+        shape = (6,) + tuple(xx.shape)
+
+        def empty():
+            return np.empty(shape) if device is None else torch.empty(shape, **f64)
+
+        self.X, self.Y, self.Z = empty(), empty(), empty()
+        self.lon, self.lat = empty(), empty()
+        self.ex_lon, self.ex_lat = empty(), empty()
+        self.ey_lon, self.ey_lat = empty(), empty()
         for p in range(6):
             X, Y, Z = _rotate(p, c0, c1, c2)
             ex = _rotate(p, *ex0)
             ey = _rotate(p, *ey0)
-            lon = np.arctan2(Y, X)
-            lat = np.arctan2(Z, np.hypot(X, Y))
-            sl, cl, st, ct = np.sin(lon), np.cos(lon), np.sin(lat), np.cos(lat)
+            h = xp.hypot(X, Y)
+            lon = xp.arctan2(Y, X)
+            lat = xp.arctan2(Z, h)
+            # unit lat-lon vectors from the position itself (|X,Y,Z| = 1): sin lon = Y/h,
+            # cos lon = X/h, sin lat = Z, cos lat = h; no trigonometric call per point.  At a
+            # pole (h = 0, a cell centre of odd N) lon = atan2(+-0, +-0) follows the signed
+            # zeros, so those few points take sin / cos of lon itself
+            pole = h == 0.0
+            if device is None:
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    sl, cl = Y / h, X / h
+                if pole.any():
+                    sl[pole], cl[pole] = np.sin(lon[pole]), np.cos(lon[pole])
+            else:
+                sl = torch.where(pole, torch.sin(lon), Y / h)
+                cl = torch.where(pole, torch.cos(lon), X / h)
+            st, ct = Z, h
             elon = (-sl, cl, 0.0)
             elat = (-st * cl, -st * sl, ct)
             self.X[p], self.Y[p], self.Z[p], self.lon[p], self.lat[p] = X, Y, Z, lon, lat
@@ -90,7 +121,7 @@ class Grid:
 
     projection = "gnomonic_equiangular"
 
-    def __init__(self, N):
+    def __init__(self, N, device=None):
         if N < 8:
             raise ValueError("N must be >= 8 (the 4-cell boundary frames of opposite edges may not overlap)")
         self.N = N
@@ -105,12 +136,13 @@ class Grid:
         # same 1-D coordinates as src/cs_datastruct.py:272-273, 298-300
         self.x_pc = np.linspace(-a + dx / 2.0 - NG * dx, a - dx / 2.0 + NG * dx, N + 2 * NG)
         self.x_pu = np.linspace(-a - NG * dx, a + NG * dx, N + 1 + 2 * NG)
-        self.pc = PointSet(self.x_pc, self.x_pc)
-        self.pu = PointSet(self.x_pu, self.x_pc)
-        self.pv = PointSet(self.x_pc, self.x_pu)
+        self.device = device
+        self.pc = PointSet(self.x_pc, self.x_pc, device)
+        self.pu = PointSet(self.x_pu, self.x_pc, device)
+        self.pv = PointSet(self.x_pc, self.x_pu, device)
         I = slice(self.i0, self.iend)
         gi = self.pc.g[I, I]
-        self.a2 = 6.0 * float(np.sum(gi * gi))   # sum of g^2 over the 6 panel interiors
+        self.a2 = 6.0 * float((gi * gi).sum())   # sum of g^2 over the 6 panel interiors
 
     # helpers for the reference layout Q[i, j, p]
     @staticmethod

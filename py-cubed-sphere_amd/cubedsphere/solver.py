@@ -29,11 +29,26 @@ def _ptr(t):
 
 
 def _pad_last(a, width):
-    """Zero-pad the last axis of a numpy array to ``width`` columns."""
-    a = np.asarray(a)
-    out = np.zeros(a.shape[:-1] + (width,), dtype=a.dtype)
+    """Zero-pad the last axis of a numpy array or torch tensor to ``width`` columns."""
+    if torch.is_tensor(a):
+        out = torch.zeros(a.shape[:-1] + (width,), dtype=a.dtype, device=a.device)
+    else:
+        a = np.asarray(a)
+        out = np.zeros(a.shape[:-1] + (width,), dtype=a.dtype)
     out[..., :a.shape[-1]] = a
     return out
+
+
+def _host(a):
+    return a.cpu().numpy() if torch.is_tensor(a) else np.asarray(a)
+
+
+def _as_tensor(a):
+    return a if torch.is_tensor(a) else torch.from_numpy(np.ascontiguousarray(a))
+
+
+def _stack(arrs):
+    return torch.stack(arrs) if torch.is_tensor(arrs[0]) else np.stack(arrs)
 
 
 class GridTables:
@@ -49,6 +64,8 @@ class GridTables:
         ld, ldu = self.ld, self.ldu
 
         def up(a, tdt=torch.float64):
+            if torch.is_tensor(a):
+                return a.to(device=dev, dtype=tdt).contiguous()
             return torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=tdt)
 
         g0 = grid.pc.g
@@ -62,7 +79,7 @@ class GridTables:
         self.lagk = up(k, torch.int32)
         self.orient = up(orientation(N), torch.int32)
         rec = ghost_recipes(grid, ld)
-        gflat = _pad_last(g0, ld).reshape(-1)
+        gflat = _host(_pad_last(g0, ld)).reshape(-1)
         self.n_strip, self.n_corner = len(rec["strip_tgt"]), len(rec["corner_tgt"])
         self.strip_tgt, self.strip_src = up(rec["strip_tgt"], torch.int32), up(rec["strip_src"], torch.int32)
         self.strip_w = up(rec["strip_w"])
@@ -71,19 +88,21 @@ class GridTables:
         self.corner_w = up(rec["corner_w"])
         self.corner_gw = up(np.sum(rec["corner_w"] * gflat[rec["corner_gsrc"]], axis=1))
         pc, pu, pv = grid.pc, grid.pu, grid.pv
-        self.conv_pc = up(_pad_last(np.stack([pc.ex_lon, pc.ex_lat, pc.ey_lon, pc.ey_lat]), ld))
-        self.conv_pu = up(_pad_last(np.stack([pu.ex_lon, pu.ex_lat, pu.ey_lon, pu.ey_lat, pu.det]), ldu))
-        self.conv_pv = up(_pad_last(np.stack([pv.ex_lon, pv.ex_lat, pv.ey_lon, pv.ey_lat, pv.det]), ld))
+        self.conv_pc = up(_pad_last(_stack([pc.ex_lon, pc.ex_lat, pc.ey_lon, pc.ey_lat]), ld))
+        self.conv_pu = up(_pad_last(_stack([pu.ex_lon, pu.ex_lat, pu.ey_lon, pu.ey_lat, pu.det]), ldu))
+        self.conv_pv = up(_pad_last(_stack([pv.ex_lon, pv.ex_lat, pv.ey_lon, pv.ey_lat, pv.det]), ld))
 
 
 _TABLES = {}
 
 
-def grid_tables(N, dtype, device):
-    """GridTables of (N, dtype, device), built once per process and shared by every solve."""
-    key = (int(N), dtype, str(torch.device(device)))
+def grid_tables(N, dtype, device, device_grid=False):
+    """GridTables of (N, dtype, device), built once per process and shared by every solve.
+    device_grid: build the grid geometry on the device (torch FP64) instead of the host."""
+    dev = torch.device(device)
+    key = (int(N), dtype, str(dev), bool(device_grid))
     if key not in _TABLES:
-        _TABLES[key] = GridTables(Grid(N), dtype, torch.device(device))
+        _TABLES[key] = GridTables(Grid(N, device=dev if device_grid else None), dtype, dev)
     return _TABLES[key]
 
 
@@ -94,8 +113,12 @@ def interior_winds(grid, case):
     M = grid.M
     basis = case.basis()
     K = len(basis)
-    wu = np.zeros((K, 6, M, M + 1))
-    wv = np.zeros((K, 6, M + 1, M))
+    if torch.is_tensor(grid.pu.lon):
+        wu = torch.zeros((K, 6, M, M + 1), dtype=torch.float64, device=grid.pu.lon.device)
+        wv = torch.zeros((K, 6, M + 1, M), dtype=torch.float64, device=grid.pu.lon.device)
+    else:
+        wu = np.zeros((K, 6, M, M + 1))
+        wv = np.zeros((K, 6, M + 1, M))
     su = (slice(None), slice(i0, iend), slice(i0, iend + 1))
     sv = (slice(None), slice(i0, iend + 1), slice(i0, iend))
     pu, pv = grid.pu, grid.pv
@@ -107,13 +130,17 @@ def interior_winds(grid, case):
     return wu, wv
 
 
-def cfl_dt(grid, case, cfl):
+def cfl_dt(grid, case, cfl, winds=None):
     """Largest dt with max |contravariant wind| * dt / dx <= cfl over the interior edges at t=0."""
-    wu, wv = interior_winds(grid, case)
+    wu, wv = winds if winds is not None else interior_winds(grid, case)
     s = case.coefficients(0.0)
-    u = np.tensordot(s, wu, axes=1)
-    v = np.tensordot(s, wv, axes=1)
-    vmax = max(np.abs(u).max(), np.abs(v).max())
+    if torch.is_tensor(wu):
+        st = torch.as_tensor(s, dtype=torch.float64, device=wu.device)
+        u, v = torch.tensordot(st, wu, dims=1), torch.tensordot(st, wv, dims=1)
+        vmax = max(float(u.abs().max()), float(v.abs().max()))
+    else:
+        u, v = np.tensordot(s, wu, axes=1), np.tensordot(s, wv, axes=1)
+        vmax = max(np.abs(u).max(), np.abs(v).max())
     return cfl * grid.dx / vmax
 
 
@@ -142,6 +169,7 @@ class Advection:
         self.handle = None
         self.index = 0              # solve index inside the handle (a batch shares one handle)
         self.owner = None           # the Batch whose handle this solve uses, if any
+        self.stop = 0               # > 0: the kernels stop advancing this solve after `stop` steps
         self.steps_done = 0
         self._alloc()
 
@@ -171,6 +199,16 @@ class Advection:
         self.partials = torch.zeros(ntr * nparts, **f64)
         self.ctl = torch.zeros(4, device=dev, dtype=torch.int32)
 
+    def device_bytes(self):
+        """Device memory held by this solve's own buffers (tables shared per N excluded)."""
+        ts = [self.uavg, self.vavg, self.usgn, self.vsgn, self.wbu, self.wbv, self.wpc, self.coef, self.corr,
+              self.partials, self.ctl] + list(self.q)
+        return sum(t.numel() * t.element_size() for t in ts if t is not None)
+
+    def release_wind_scratch(self):
+        """Free the lat-lon wind scratch once the winds are prepared (only wind init uses it)."""
+        self.wpc = None
+
     def desc(self):
         g, tab = self.grid, self.tab
         d = _lib.SolveDesc()
@@ -188,7 +226,8 @@ class Advection:
         d.gc, d.gu, d.gv, d.rgc, d.gc64 = _ptr(tab.gc), _ptr(tab.gu), _ptr(tab.gv), _ptr(tab.rgc), _ptr(tab.gc64)
         d.lagw, d.lagk, d.orient = _ptr(tab.lagw), _ptr(tab.lagk), _ptr(tab.orient)
         d.wbu, d.wbv = _ptr(self.wbu), _ptr(self.wbv)
-        d.conv_pc, d.conv_pu, d.conv_pv, d.wpc = _ptr(tab.conv_pc), _ptr(tab.conv_pu), _ptr(tab.conv_pv), _ptr(self.wpc)
+        d.conv_pc, d.conv_pu, d.conv_pv = _ptr(tab.conv_pc), _ptr(tab.conv_pu), _ptr(tab.conv_pv)
+        d.wpc = _ptr(self.wpc)      # null once the winds are prepared (scratch released)
         d.n_strip, d.n_corner = tab.n_strip, tab.n_corner
         d.strip_tgt, d.strip_src, d.strip_w, d.strip_gw = (_ptr(tab.strip_tgt), _ptr(tab.strip_src), _ptr(tab.strip_w),
                                                            _ptr(tab.strip_gw))
@@ -196,6 +235,7 @@ class Advection:
                                                                _ptr(tab.corner_w), _ptr(tab.corner_gw))
         d.coef, d.corr, d.partials, d.ctl = _ptr(self.coef), _ptr(self.corr), _ptr(self.partials), _ptr(self.ctl)
         d.tile = self.tile
+        d.stop = self.stop
         return d
 
     def _stream(self):
@@ -218,14 +258,16 @@ class Advection:
         control words (the device part of init_vars_adv; no handle yet)."""
         g = self.grid
         i0, iend = g.i0, g.iend
-        wu, wv = interior_winds(g, self.case)
+        pre = getattr(self, "basis_winds", None)       # interior basis winds computed with dt
+        wu, wv = pre if pre is not None else interior_winds(g, self.case)
+        self.basis_winds = None
         ldu, ld = self.tab.ldu, self.tab.ld
-        self.wbu.copy_(torch.from_numpy(_pad_last(wu, ldu)))
-        self.wbv.copy_(torch.from_numpy(_pad_last(wv, ld)))
+        self.wbu.copy_(_as_tensor(_pad_last(wu, ldu)))
+        self.wbv.copy_(_as_tensor(_pad_last(wv, ld)))
         I = slice(i0, iend)
-        q0 = np.stack([self.case.q(n, g.pc.lon[:, I, I], g.pc.lat[:, I, I], 0.0) for n in range(self.case.ntr)])
+        q0 = _stack([self.case.q(n, g.pc.lon[:, I, I], g.pc.lat[:, I, I], 0.0) for n in range(self.case.ntr)])
         self.q[0].zero_()
-        self.q[0][:, :, I, I] = torch.from_numpy(q0).to(self.q[0].dtype)
+        self.q[0][:, :, I, I] = _as_tensor(q0).to(self.q[0].device, self.q[0].dtype)
         self.q[1].zero_()
         self.ctl.zero_()
         self.corr.zero_()

@@ -25,7 +25,7 @@ def _b6(a):
     return np.broadcast_to(a, (6,) + a.shape)
 
 
-@pytest.mark.parametrize("N", [8, 16, 40])
+@pytest.mark.parametrize("N", [8, 16, 17, 40])
 def test_grid_tables_match_oracle(N):
     g, o = Grid(N), ogrid.CubedSphere(N)
     assert np.array_equal(R(g.pc.X), o.pc.X) and np.array_equal(R(g.pu.Z), o.pu.Z)
