@@ -1,15 +1,18 @@
 // libcsadv: MI355X-native dimension-split finite-volume advection on the cubed sphere.
 //
-// Kernels (all launched on the caller's stream, none allocates):
-//   k_ghost_edges / k_ghost_corners   duo-grid ghost fill (src/interpolation.py:154-315)
-//   k_step<T,RECON,TX,TY>             fused per-tile step: both directional passes of the
-//                                     split, PPM edges, upwind fluxes, update, mass-fixer
-//                                     partial sums (src/discrete_operators.py:18-101,
-//                                     src/flux.py, src/reconstruction_1d.py)
-//   k_step_reduce                     deterministic MF-PR sum -> pending correction
+// Kernels (all launched on the caller's stream; none allocates):
+//   k_ghost_edges / k_ghost_corners   full 4-layer duo-grid ghost fill (src/interpolation.py:154-315),
+//                                     used by csadv_ghost_fill and for the wind basis
+//   k_ghost                           per-step ghost fill of the 3 layers a step reads, from
+//                                     flattened recipes, plus the MF-PR sum of the last step
+//   k_step<T,RECON,RK2>               the step: both directional passes of the split, PPM edges,
+//                                     upwind fluxes, update and the mass-fixer partial sums of one
+//                                     (solve, panel, tracer, column strip, row segment) per
+//                                     wavefront (src/discrete_operators.py:18-101, src/flux.py,
+//                                     src/reconstruction_1d.py)
 //   k_wind_e2c / k_wind_c2e           duo-grid vector ghost fill (src/interpolation.py:347-532)
 //   k_wind_avg                        time-averaged winds (src/averaged_velocity.py:14-62)
-//   k_mass / k_norms / k_finalize     diagnostics and state materialisation
+//   k_settle / k_finalize / k_clear_corr, k_mass / k_norms   state read-out and diagnostics
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -154,27 +157,47 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
   const csadv_solve_desc &D = S.d;
   const int ncell = D.n_strip + D.n_corner;
   if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
+  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = idx < ncell;
+  const bool strip = idx < D.n_strip;
+  const int c = strip ? idx : idx - D.n_strip;
+  // the recipe of this cell does not depend on the step: request all of it first, so that
+  // its latency overlaps the read of the step counter and the partial sums below
+  int s16[16];
+  double w16[16];
+  int tgt = 0;
+  double gw = 0.0;
+  if (active) {
+    if (strip) {
+      const int4 s4 = reinterpret_cast<const int4 *>(D.strip_src)[c];
+      s16[0] = s4.x; s16[1] = s4.y; s16[2] = s4.z; s16[3] = s4.w;
+      const double2 *w2 = reinterpret_cast<const double2 *>(D.strip_w + 4 * c);
+      const double2 a = w2[0], b = w2[1];
+      w16[0] = a.x; w16[1] = a.y; w16[2] = b.x; w16[3] = b.y;
+      tgt = D.strip_tgt[c];
+      gw = D.strip_gw[c];
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const int4 s4 = reinterpret_cast<const int4 *>(D.corner_src + 16 * c)[l];
+        s16[4 * l] = s4.x; s16[4 * l + 1] = s4.y; s16[4 * l + 2] = s4.z; s16[4 * l + 3] = s4.w;
+      }
+      const double2 *w2 = reinterpret_cast<const double2 *>(D.corner_w + 16 * c);
+#pragma unroll
+      for (int l = 0; l < 8; ++l) {
+        const double2 a = w2[l];
+        w16[2 * l] = a.x; w16[2 * l + 1] = a.y;
+      }
+      tgt = D.corner_tgt[c];
+      gw = D.corner_gw[c];
+    }
+  }
   const int steps = D.ctl[1];
   if (D.stop > 0 && steps >= D.stop) {      // solve finished: k_step reads ctl[2] and skips it too
     if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
     return;
   }
   T *Q = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0);
-  const int idx = blockIdx.x * BLOCK + threadIdx.x;
-  const bool active = idx < ncell;
-  const bool strip = idx < D.n_strip;
-  const int c = strip ? idx : idx - D.n_strip;
-  const int32_t *src = strip ? D.strip_src + 4 * c : D.corner_src + 16 * c;
-  const double *w = strip ? D.strip_w + 4 * c : D.corner_w + 16 * c;
-  // source offsets are the same for every tracer
-  int s16[16];
-  if (active) {
-    const int nq = strip ? 1 : 4;
-    for (int l = 0; l < nq; ++l) {
-      const int4 s4 = reinterpret_cast<const int4 *>(src)[l];
-      s16[4 * l] = s4.x; s16[4 * l + 1] = s4.y; s16[4 * l + 2] = s4.z; s16[4 * l + 3] = s4.w;
-    }
-  }
   for (int tr = 0; tr < D.ntr; ++tr) {
     T *Qt = Q + tr * S.tstride;
     // weighted sum of the sources first (in recipe order), so that its loads overlap the
@@ -183,23 +206,23 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
     if (active) {
       if (strip) {
         const T v0 = Qt[s16[0]], v1 = Qt[s16[1]], v2 = Qt[s16[2]], v3 = Qt[s16[3]];
-        acc += (double)v0 * w[0];
-        acc += (double)v1 * w[1];
-        acc += (double)v2 * w[2];
-        acc += (double)v3 * w[3];
+        acc += (double)v0 * w16[0];
+        acc += (double)v1 * w16[1];
+        acc += (double)v2 * w16[2];
+        acc += (double)v3 * w16[3];
       } else {
         T v[16];
 #pragma unroll
         for (int l = 0; l < 16; ++l) v[l] = Qt[s16[l]];
 #pragma unroll
-        for (int l = 0; l < 16; ++l) acc += (double)v[l] * w[l];
+        for (int l = 0; l < 16; ++l) acc += (double)v[l] * w16[l];
       }
     }
     const double cf = pending_corr(S, tr, red);
     if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && D.ctl[3]) D.corr[tr] = cf;
     if (!active) continue;
-    acc += cf * (strip ? D.strip_gw[c] : D.corner_gw[c]);
-    Qt[strip ? D.strip_tgt[c] : D.corner_tgt[c]] = (T)acc;
+    acc += cf * gw;
+    Qt[tgt] = (T)acc;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
 }
@@ -218,7 +241,7 @@ __global__ void k_settle(Solve *solves, int first) {
   if (threadIdx.x == 0) D.ctl[3] = 0;
 }
 
-// Work item = (solve, panel, strip, segment): one wavefront marches down strip_w interior
+// Work item = (solve, panel, tracer, strip, segment): one wavefront marches down strip_w interior
 // columns (lane l <-> column ti-3+l, so the 3-cell halo of each side rides in the outer
 // lanes) over seg_l interior rows.  Iteration y:
 //   * column stage at y-edge e = y-3 from the 6-row windows (rows y-6 .. y-1): y-fluxes of
@@ -376,7 +399,7 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   if (wk.x < 0) return;              // padding item of the XCD-banded work list
   const Solve &S = solves[wk.x];
   const csadv_solve_desc &D = S.d;
-  const int p = wk.y;
+  const int p = wk.y & 7, tr = wk.y >> 3;    // panel, tracer
   const int N = D.N;
   const int ti = NG + wk.z * S.strip_w, tj = NG + wk.w * S.seg_l;
   const int iend = N + NG;
@@ -385,7 +408,7 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   if (D.stop > 0 && step >= D.stop) return;   // solve finished
   const int item = (p * S.segs + wk.w) * S.strips + wk.z;
   const bool mfpr = D.mf == 3;
-  if (item == 0 && lane == 0) {      // hand the step over to the next kernels of the stream
+  if (item == 0 && tr == 0 && lane == 0) {   // hand the step over to the next kernels of the stream
     D.ctl[1] = step + 1;
     if (mfpr) D.ctl[3] = 1;
   }
@@ -419,15 +442,13 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   m.gc64 = D.gc64;
   const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
   T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
-  for (int tr = 0; tr < D.ntr; ++tr) {
-    m.Q = Qin + tr * S.tstride + p * S.pstride;
-    m.Qo = Qout + tr * S.tstride + p * S.pstride;
-    m.cf = D.corr ? D.corr[tr] : 0.0;
-    m.run();
-    if (mfpr) {
-      const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
-      if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
-    }
+  m.Q = Qin + tr * S.tstride + p * S.pstride;
+  m.Qo = Qout + tr * S.tstride + p * S.pstride;
+  m.cf = D.corr ? D.corr[tr] : 0.0;
+  m.run();
+  if (mfpr) {
+    const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
+    if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
   }
 }
 
@@ -766,8 +787,8 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";
     return CSADV_E_ARG;
   }
-  if (((uintptr_t)d.strip_src | (uintptr_t)d.corner_src) % 16 != 0) {
-    msg = "strip_src and corner_src must be 16-byte aligned";
+  if (((uintptr_t)d.strip_src | (uintptr_t)d.corner_src | (uintptr_t)d.strip_w | (uintptr_t)d.corner_w) % 16 != 0) {
+    msg = "strip_src, corner_src, strip_w and corner_w must be 16-byte aligned";
     return CSADV_E_ARG;
   }
   if (d.stop < 0) { msg = "stop must be >= 0"; return CSADV_E_RANGE; }
@@ -928,9 +949,13 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
     h->host.push_back(S);
     // segment rows are cut into kXcds contiguous bands, one per XCD: the band's halo rows and
     // its share of the panel-invariant metric then stay in that XCD's L2 for all six panels
+    // (the tracers of one segment are neighbours in a band, so they share its wind and
+    // metric rows in L2; the work item packs panel and tracer as p + 8 * tracer)
     for (int sg = 0; sg < S.segs; ++sg)
       for (int p = 0; p < 6; ++p)
-        for (int sx = 0; sx < S.strips; ++sx) band[sg * kXcds / S.segs].push_back(make_int4(s, p, sx, sg));
+        for (int sx = 0; sx < S.strips; ++sx)
+          for (int tr = 0; tr < solves[s].ntr; ++tr)
+            band[sg * kXcds / S.segs].push_back(make_int4(s, p + 8 * tr, sx, sg));
     FieldSet F{};
     F.base[0] = solves[s].q0;
     F.base[1] = solves[s].q1;
