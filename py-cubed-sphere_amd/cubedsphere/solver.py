@@ -333,7 +333,7 @@ class Advection:
 
     def norms(self, tracer=0, t=None):
         """(Linf, L1, L2) against the exact solution (src/errors.py:99-113)."""
-        qe = torch.from_numpy(np.ascontiguousarray(self.exact(tracer, t))).to(self.device, torch.float64)
+        qe = _as_tensor(self.exact(tracer, t)).to(self.device, torch.float64).contiguous()
         out = (ctypes.c_double * 3)()
         _lib.check(self.lib.csadv_norms(self.handle, self.index, tracer, _ptr(qe), out, self._stream()), self.handle,
                    "csadv_norms")

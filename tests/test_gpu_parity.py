@@ -186,3 +186,19 @@ def test_every_tile_shape_matches_oracle(tile):
     sol.step(nsteps)
     grid, sim = run_oracle(N, case, dt, nsteps)
     assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
+
+
+def test_drop_in_driver_matches_oracle_run(tmp_path):
+    """adv_sphere from the par files (test case 5, tc = 1) against the oracle's run: same
+    final error norms and mass conservation (src/advection_sphere.py:13-61)."""
+    from cubedsphere import advection
+    (tmp_path / "configuration.par").write_text("a\nb\n16\nc\n2\nd\n0\ne\n0\nf\n5\ng\n1\n")
+    (tmp_path / "advection.par").write_text(
+        # Tf, dt, ic, vf, tc, recon, dp, opsplit, et, mt, mf (one value every other line)
+        "\n".join(["x", "y", "0.25", "#", "0.025", "#", "2", "#", "1", "#", "1", "#", "3", "#", "1", "#", "1", "#",
+                   "3", "#", "1", "#", "3"]) + "\n")
+    linf, l1, l2 = advection.main(str(tmp_path))
+    grid, sim = ots.make(16, {"ic": 2, "vf": 1}, 0.025)
+    ref = ots.run(grid, sim, 10)
+    for a, b in zip((linf, l1, l2), ref):
+        assert abs(a - b) <= 1e-10 * abs(b)

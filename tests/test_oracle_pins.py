@@ -319,3 +319,35 @@ def test_whole_step_converges():
         errs.append(ots.run(g, s, nst)[2])
     assert errs[0] < 2e-3
     assert errs[0] / errs[1] > 2.5, errs
+
+
+# ------------------------------------------------------------------ vector ghost fill
+def test_vector_ghost_fill_converges_to_analytic_wind():
+    """src/interpolation.py:347-532: the contravariant winds the vector fill writes outside the
+    interior (x-edge ghost rows, y-edge ghost columns and the RK2 departure edges i0-1, iend+1)
+    converge to the analytic wind at those points; a wrong orientation, sign or stencil in any
+    strip leaves an O(1) error there instead."""
+    from oracle import cases as ocases
+    from oracle.grid import latlon_to_contravariant
+    errs = []
+    for N in (16, 32, 64):
+        grid, sim = ots.make(N, {"ic": 2, "vf": 1}, 0.01)
+        ots.init_vars_adv(grid, sim)
+        i0, iend = grid.i0, grid.iend
+        ul, vl = ocases.case_wind(sim.case, grid.pu.lon, grid.pu.lat, 0.0)
+        uex, _ = latlon_to_contravariant(ul, vl, grid.prod_ex_elon_pu, grid.prod_ex_elat_pu, grid.prod_ey_elon_pu,
+                                         grid.prod_ey_elat_pu, grid.determinant_ll2contra_pu)
+        ul, vl = ocases.case_wind(sim.case, grid.pv.lon, grid.pv.lat, 0.0)
+        _, vex = latlon_to_contravariant(ul, vl, grid.prod_ex_elon_pv, grid.prod_ex_elat_pv, grid.prod_ey_elon_pv,
+                                         grid.prod_ey_elat_pv, grid.determinant_ll2contra_pv)
+        J, E = slice(i0, iend), slice(i0, iend + 1)
+        ghost = list(range(0, i0)) + list(range(iend, grid.N + 2 * grid.ngl))
+        rk2 = [i0 - 1, iend + 1]
+        U, V = sim.U_pu.ucontra, sim.U_pv.vcontra
+        errs.append(max(np.abs(U[E][:, ghost] - uex[E][:, ghost]).max(),
+                        np.abs(V[:, E][ghost] - vex[:, E][ghost]).max(),
+                        np.abs(U[rk2][:, J] - uex[rk2][:, J]).max(),
+                        np.abs(V[:, rk2][J] - vex[:, rk2][J]).max()))
+    errs = np.array(errs)
+    assert errs[-1] < 1e-5
+    assert np.all(errs[:-1] / errs[1:] > 7.0), errs          # third order or better
