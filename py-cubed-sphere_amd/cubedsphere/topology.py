@@ -200,3 +200,35 @@ def ghost_recipes(grid, ld, depth=3):
     out["corner_gsrc"] = np.array(cgsrc, dtype=np.int32)
     out["corner_w"] = np.array(cw, dtype=np.float64)
     return out
+
+
+_OUT_SIGN = {E: 1, W: -1, N_: 1, S: -1}      # outward normal of a side, in the panel's own axes
+
+
+def cube_edge_pairs(N):
+    """The 24 panel sides and what lies across each cube edge, for the cross-panel steps of
+    ET-PL07 (src/edges_treatment.py:31-178) and MF-AF (:231-278).
+
+    Returns int32 [6][4][4]: for panel p and side s, (q, s_q, rev, sign):
+      q     the neighbour panel;  s_q its side on the shared edge;
+      rev   1 if the along-edge index runs backwards on the neighbour (interior index k of
+            ours is N-1-k of the neighbour's), else 0;
+      sign  +1 / -1: the neighbour's axis across the edge points the same / opposite way as
+            ours (edge values swap q_L <-> q_R, fluxes change sign).
+    Derived from the halo orientation maps: the neighbour cell next to our ghost at depth 0
+    is the neighbour's boundary cell, and the direction in which depth grows tells its side.
+    """
+    o = orientation(N)
+    out = np.zeros((6, 4, 4), dtype=np.int32)
+    for p in range(6):
+        for s in (E, W, N_, S):
+            q, ib, id_, ir, jb, jd, jr = (int(v) for v in o[p, s])
+            if id_ != 0:
+                s_q = W if id_ > 0 else E          # depth grows into q from its west / east side
+                along = jr
+            else:
+                s_q = S if jd > 0 else N_
+                along = ir
+            inward = {W: 1, E: -1, S: 1, N_: -1}[s_q]
+            out[p, s] = (q, s_q, 1 if along < 0 else 0, _OUT_SIGN[s] * inward)
+    return out

@@ -209,3 +209,195 @@ def test_create_rejects_invalid_descriptor_without_gpu_work():
     assert rc == -1
     assert b"opsplit" in lib.csadv_last_error(h)
     lib.csadv_destroy(h)
+
+
+# ------------------------------------------------------------------ cube-edge pairings
+def _side_index(side, N, i0):
+    """(array axis, boundary index, ghost index) of a side in reference layout [i, j, p]."""
+    iend = i0 + N
+    return {0: (0, iend - 1, iend), 1: (0, i0, i0 - 1), 2: (1, iend - 1, iend), 3: (1, i0, i0 - 1)}[side]
+
+
+def _along(a, axis, idx, p, i0, N):
+    """The N interior values along the edge at index idx of axis (other axis interior)."""
+    I = slice(i0, i0 + N)
+    return a[idx, I, p] if axis == 0 else a[I, idx, p]
+
+
+def _set_along(a, axis, idx, p, i0, N, v):
+    I = slice(i0, i0 + N)
+    if axis == 0:
+        a[idx, I, p] = v
+    else:
+        a[I, idx, p] = v
+
+
+def _generic_parabola_average_and_ghosts(pxL, pxR, pyL, pyR, N, i0, tab):
+    """What the device does for ET-PL07 from the pair table: average every cube-edge value
+    with the one across the edge, then copy each neighbour boundary parabola into the ghost."""
+    arrs = {0: (pxL, pxR), 1: (pxL, pxR), 2: (pyL, pyR), 3: (pyL, pyR)}
+    def bval(p, s):                          # boundary edge value facing the cube edge
+        L, R = arrs[s]
+        axis, b, _ = _side_index(s, N, i0)
+        return _along(R if s in (0, 2) else L, axis, b, p, i0, N)
+    new = {}
+    for p in range(6):
+        for s in range(4):
+            q, sq, rev, sg = (int(v) for v in tab[p, s])
+            nb = bval(q, sq)
+            nb = nb[::-1] if rev else nb
+            new[(p, s)] = (bval(p, s) + nb) * 0.5
+    for (p, s), v in new.items():
+        L, R = arrs[s]
+        axis, b, _ = _side_index(s, N, i0)
+        _set_along(R if s in (0, 2) else L, axis, b, p, i0, N, v)
+    ghosts = {}
+    for p in range(6):
+        for s in range(4):
+            q, sq, rev, sg = (int(v) for v in tab[p, s])
+            Lq, Rq = arrs[sq]
+            axis, b, _ = _side_index(sq, N, i0)
+            l, r = _along(Lq, axis, b, q, i0, N), _along(Rq, axis, b, q, i0, N)
+            if rev:
+                l, r = l[::-1], r[::-1]
+            ghosts[(p, s)] = (l, r) if sg > 0 else (r, l)
+    for (p, s), (l, r) in ghosts.items():
+        L, R = arrs[s]
+        axis, _, gidx = _side_index(s, N, i0)
+        _set_along(L, axis, gidx, p, i0, N, l)
+        _set_along(R, axis, gidx, p, i0, N, r)
+
+
+def test_cube_edge_pairs_reproduce_et_pl07_averaging_and_ghost_copies():
+    from cubedsphere.topology import cube_edge_pairs
+    from oracle import edges as oedges
+    from oracle.recon import Parabola
+    from types import SimpleNamespace
+    N = 12
+    M, i0 = N + 8, 4
+    rng = np.random.default_rng(3)
+    g = SimpleNamespace(i0=i0, j0=i0, iend=i0 + N, jend=i0 + N)
+    px, py = Parabola(M, "PPM-PL07", "x"), Parabola(M, "PPM-PL07", "y")
+    for a in (px.q_L, px.q_R, py.q_L, py.q_R):
+        a[:] = rng.standard_normal(a.shape)
+    ref = [a.copy() for a in (px.q_L, px.q_R, py.q_L, py.q_R)]
+    oedges.average_parabola_cube_edges(px, py, g)
+    # the ghost copies of edges_extrapolation follow the averaging; run them through the
+    # whole routine on arrays whose boundary cells eq. 47 / 49 leave as they are
+    mine = [a.copy() for a in ref]
+    _generic_parabola_average_and_ghosts(*mine, N, i0, cube_edge_pairs(N))
+    I = slice(i0, i0 + N)
+    for got, want in zip(mine, (px.q_L, px.q_R, py.q_L, py.q_R)):
+        # boundary rows / columns after averaging
+        assert np.array_equal(got[i0, I], want[i0, I]) and np.array_equal(got[i0 + N - 1, I], want[i0 + N - 1, I])
+        assert np.array_equal(got[I, i0], want[I, i0]) and np.array_equal(got[I, i0 + N - 1], want[I, i0 + N - 1])
+
+
+def test_cube_edge_pairs_reproduce_et_pl07_ghost_parabolas():
+    from cubedsphere.topology import cube_edge_pairs
+    from oracle import edges as oedges
+    from oracle.recon import Parabola
+    from types import SimpleNamespace
+    N = 12
+    M, i0 = N + 8, 4
+    rng = np.random.default_rng(4)
+    g = SimpleNamespace(i0=i0, j0=i0, iend=i0 + N, jend=i0 + N)
+    Qx, Qy = rng.standard_normal((M, M, 6)), rng.standard_normal((M, M, 6))
+    px, py = Parabola(M, "PPM-PL07", "x"), Parabola(M, "PPM-PL07", "y")
+    oedges.edges_extrapolation(Qx, Qy, px, py, g, "PPM-0")      # eq. 47 / 49, averaging, ghosts
+    # the same from the pair table: eq. 47 / 49 by the oracle's formulas, then the table
+    px2, py2 = Parabola(M, "PPM-PL07", "x"), Parabola(M, "PPM-PL07", "y")
+    J = slice(i0, i0 + N)
+    iend = i0 + N
+    px2.q_L[i0, J] = 1.5 * Qx[i0, J] - 0.5 * Qx[i0 + 1, J]
+    px2.q_R[iend - 1, J] = 1.5 * Qx[iend - 1, J] - 0.5 * Qx[iend - 2, J]
+    py2.q_L[J, i0] = 1.5 * Qy[J, i0] - 0.5 * Qy[J, i0 + 1]
+    py2.q_R[J, iend - 1] = 1.5 * Qy[J, iend - 1] - 0.5 * Qy[J, iend - 2]
+    px2.q_R[i0, J] = (3.0 * Qx[i0, J] + 11.0 * Qx[i0 + 1, J] - 2.0 * (Qx[i0 + 2, J] - Qx[i0, J])) / 14.0
+    px2.q_L[iend - 1, J] = (3.0 * Qx[iend - 1, J] + 11.0 * Qx[iend - 2, J] - 2.0 * (Qx[iend - 3, J] - Qx[iend - 1, J])) / 14.0
+    py2.q_R[J, i0] = (3.0 * Qy[J, i0] + 11.0 * Qy[J, i0 + 1] - 2.0 * (Qy[J, i0 + 2] - Qy[J, i0])) / 14.0
+    py2.q_L[J, iend - 1] = (3.0 * Qy[J, iend - 1] + 11.0 * Qy[J, iend - 2] - 2.0 * (Qy[J, iend - 3] - Qy[J, iend - 1])) / 14.0
+    _generic_parabola_average_and_ghosts(px2.q_L, px2.q_R, py2.q_L, py2.q_R, N, i0, cube_edge_pairs(N))
+    for a, b in ((px.q_L, px2.q_L), (px.q_R, px2.q_R)):
+        for idx in (i0 - 1, i0, iend - 1, iend):
+            assert np.array_equal(a[idx, J], b[idx, J]), idx
+    for a, b in ((py.q_L, py2.q_L), (py.q_R, py2.q_R)):
+        for idx in (i0 - 1, i0, iend - 1, iend):
+            assert np.array_equal(a[J, idx], b[J, idx]), idx
+
+
+def test_cube_edge_pairs_reproduce_mf_af():
+    from cubedsphere.topology import cube_edge_pairs
+    from oracle import edges as oedges
+    from oracle.recon import Parabola
+    from types import SimpleNamespace
+    N = 10
+    M, i0 = N + 8, 4
+    iend = i0 + N
+    rng = np.random.default_rng(5)
+    g = SimpleNamespace(i0=i0, j0=i0, iend=iend, jend=iend)
+    px, py = Parabola(M, "PPM-PL07", "x"), Parabola(M, "PPM-PL07", "y")
+    px.f_upw[:] = rng.standard_normal(px.f_upw.shape)
+    py.f_upw[:] = rng.standard_normal(py.f_upw.shape)
+    fx, fy = px.f_upw.copy(), py.f_upw.copy()
+    oedges.average_flux_cube_edges(px, py, g)
+    tab = cube_edge_pairs(N)
+    J = slice(i0, iend)
+    edge = {0: (fx, 0, iend), 1: (fx, 0, i0), 2: (fy, 1, iend), 3: (fy, 1, i0)}
+    def val(p, s):
+        a, axis, e = edge[s]
+        return a[e, J, p] if axis == 0 else a[J, e, p]
+    new = {}
+    for p in range(6):
+        for s in range(4):
+            q, sq, rev, sg = (int(v) for v in tab[p, s])
+            nb = val(q, sq)[::-1] if rev else val(q, sq)
+            new[(p, s)] = 0.5 * val(p, s) + 0.5 * (sg * nb)
+    for (p, s), v in new.items():
+        a, axis, e = edge[s]
+        if axis == 0:
+            a[e, J, p] = v
+        else:
+            a[J, e, p] = v
+    assert np.array_equal(fx, px.f_upw) and np.array_equal(fy, py.f_upw)
+
+
+def test_orientation_reproduces_adjacent_panel_copy_halo():
+    """ET-S72 / ET-PL07 halo (src/interpolation.py:320-340) from the orientation maps: Qy's
+    x-ghost columns and Qx's y-ghost rows are the neighbours' cells, read from Qx or Qy as
+    src/halo_data.py:191-400 selects, all sources read before any ghost is written."""
+    from oracle.interp import ghost_cells_adjacent_panels
+    from types import SimpleNamespace
+    N = 10
+    M, i0 = N + 8, 4
+    iend = i0 + N
+    rng = np.random.default_rng(6)
+    g = SimpleNamespace(i0=i0, j0=i0, iend=iend, jend=iend)
+    o = orientation(N)
+    for same in (True, False):
+        Qx = rng.standard_normal((M, M, 6))
+        Qy = Qx if same else rng.standard_normal((M, M, 6))
+        ref_x, ref_y = Qx.copy(), (Qx.copy() if same else Qy.copy())
+        ghost_cells_adjacent_panels(ref_x, ref_x if same else ref_y, g)
+        src_x, src_y = Qx.copy(), Qy.copy()
+        out_x = Qx.copy()
+        out_y = out_x if same else Qy.copy()
+        # sides E, W fill Qy's columns, N, S fill Qx's rows; panels 4, 5 read E/W from Qx,
+        # panels 1, 3 read N/S from Qy (the _WE / _NS selections)
+        writes = []
+        for p in range(6):
+            for side in (0, 1, 2, 3):
+                use_x = (p in (4, 5)) if side in (0, 1) else (p not in (1, 3))
+                src = src_x if use_x else src_y
+                for d in range(4):
+                    for r in range(M):
+                        q, i, j = halo_source(o, p, side, d, r)
+                        ti, tj = {0: (iend + d, r), 1: (i0 - 1 - d, r), 2: (r, iend + d), 3: (r, i0 - 1 - d)}[side]
+                        writes.append((side, p, ti, tj, src[i, j, q]))
+        for side in (0, 1, 2, 3):                 # x-ghost columns first, then y-ghost rows
+            for s_, p, ti, tj, v in writes:
+                if s_ == side:
+                    (out_y if side in (0, 1) else out_x)[ti, tj, p] = v
+        assert np.array_equal(out_x, ref_x)
+        if not same:
+            assert np.array_equal(out_y, ref_y)
