@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CSADV_ABI_VERSION 2
+#define CSADV_ABI_VERSION 3
 
 enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
 
@@ -44,9 +44,12 @@ enum csadv_status {
  *   recon 1 PPM-0 | 2 PPM-CW84 | 3 PPM-PL07 | 4 PPM-L04
  *   dp    1 RK1   | 2 RK2
  *   split 1 AVLT  | 2 L04      | 3 PL07
- *   et    3 ET-DG (1 ET-S72 and 2 ET-PL07 are rejected by this build: CSADV_E_ARG)
+ *   et    1 ET-S72 | 2 ET-PL07 | 3 ET-DG
  *   mt    1 MT-0  | 2 MT-PL07   (valid pairs: split 1|2 with mt 1, split 3 with mt 2)
- *   mf    1 none  | 3 MF-PR     (2 MF-AF is rejected by this build: CSADV_E_ARG)   */
+ *   mf    1 none  | 2 MF-AF     | 3 MF-PR
+ * ET-DG without MF-AF runs the fused step kernel; ET-S72, ET-PL07 and MF-AF run the general
+ * step (a kernel per stage), which needs the work buffer.  All solves of one handle take
+ * the same one of the two.                                                             */
 typedef struct csadv_solve_desc {
   int32_t N, ntr, dtype;
   int32_t ld, ldu;                 /* row strides (elements) of cell / x-edge arrays   */
@@ -91,6 +94,7 @@ typedef struct csadv_solve_desc {
                                       has completed `stop` steps (a batch can then run a fixed
                                       number of steps while its solves end at their own step);
                                       0 = no limit                                           */
+  void *work;                      /* general step only: scratch of csadv_work_bytes() bytes */
 } csadv_solve_desc;
 
 typedef struct csadv_s csadv_t;
@@ -103,6 +107,11 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
 
 /* Number of FP64 partial sums the step writes per tracer (size of desc.partials / ntr). */
 int csadv_partials_per_tracer(int32_t N, int32_t tile);
+
+/* Bytes of the work buffer the general step (ET-S72, ET-PL07, MF-AF) needs for one solve of
+ * size N and dtype (written to *bytes).  No counterpart in the reference (its scratch arrays
+ * are members of the simulation object, src/cs_datastruct.py:635-684). */
+int csadv_work_bytes(int32_t N, int32_t dtype, int64_t *bytes);
 
 /* Rows per wavefront segment of a tile shape (tile = 0 .. count-1), or CSADV_E_ARG past the
  * last shape: the tile ids an autotuner may try.  No counterpart in the reference. */

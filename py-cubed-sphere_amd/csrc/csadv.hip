@@ -709,6 +709,8 @@ __global__ void k_norms(const Solve *solves, int solve, int tracer, const double
   }
 }
 
+#include "csadv_general.hip"
+
 // ------------------------------------------------------------------ host side
 struct csadv_s {
   std::vector<csadv_solve_desc> desc;
@@ -722,6 +724,7 @@ struct csadv_s {
   int dtype = 0;
   int recon = 3;
   bool rk2 = false;               // dp = 2 (separate upwind-side winds)
+  bool general = false;           // ET-S72 / ET-PL07 / MF-AF: the general step
   double *red = nullptr;          // small reduction scratch (mass / norms)
   std::vector<int> done;          // host copy of the steps completed per solve (ctl[1])
   std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
@@ -771,8 +774,16 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "invalid (opsplit, mt) pair: valid pairs are (1|2, 1) and (3, 2)";
     return CSADV_E_ARG;
   }
-  if (d.et != 3) { msg = "only et=3 (ET-DG) is implemented by this library"; return CSADV_E_ARG; }
-  if (d.mf != 1 && d.mf != 3) { msg = "only mf=1 (none) and mf=3 (MF-PR) are implemented"; return CSADV_E_ARG; }
+  if (d.et < 1 || d.et > 3) { msg = "et must be 1 (ET-S72), 2 (ET-PL07) or 3 (ET-DG)"; return CSADV_E_ARG; }
+  if (d.mf < 1 || d.mf > 3) { msg = "mf must be 1 (none), 2 (MF-AF) or 3 (MF-PR)"; return CSADV_E_ARG; }
+  if ((d.et != 3 || d.mf == 2) && !d.work) {
+    msg = "ET-S72, ET-PL07 and MF-AF run the general step, which needs desc.work (csadv_work_bytes)";
+    return CSADV_E_ARG;
+  }
+  if ((d.et != 3 || d.mf == 2) && (d.ld != (M + 3) / 4 * 4 || d.ldu != (M + 4) / 4 * 4)) {
+    msg = "the general step expects ld = round_up(N+8, 4) and ldu = round_up(N+9, 4) (the csadv_work_bytes layout)";
+    return CSADV_E_ARG;
+  }
   if (d.nbasis < 1 || d.nbasis > 8) { msg = "nbasis must be 1..8"; return CSADV_E_RANGE; }
   if (d.unsteady && d.max_steps < 1) { msg = "unsteady winds need a coefficient table"; return CSADV_E_ARG; }
   if (!d.q0 || !d.q1 || !d.uavg || !d.vavg || !d.gc || !d.gu || !d.gv || !d.rgc || !d.gc64 || !d.lagw ||
@@ -829,6 +840,8 @@ hipError_t launch_step_dt(csadv_t *h, hipStream_t st) {
     default: return launch_step_r<T, 3>(h, st);
   }
 }
+
+int gp_blocks(int64_t total) { return (int)std::min<int64_t>(4096, (total + BLOCK - 1) / BLOCK); }
 
 void drop_graph(csadv_t *h) {
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
@@ -890,6 +903,16 @@ int csadv_partials_per_tracer(int32_t N, int32_t tile) {
   return 6 * ns * nsg;
 }
 
+int csadv_work_bytes(int32_t N, int32_t dtype, int64_t *bytes) {
+  if (!bytes || N < 8 || N > 8192 || (dtype != CSADV_F64 && dtype != CSADV_F32)) return CSADV_E_ARG;
+  const int M = N + 2 * NG;
+  const int ld = (M + 3) / 4 * 4, ldu = (M + 4) / 4 * 4;
+  const gp::Work w = gp::layout(N, ld, ldu);
+  const int64_t s = dtype == CSADV_F64 ? 8 : 4;
+  *bytes = w.total * s + (int64_t)gp::kRedGen * 8;
+  return CSADV_OK;
+}
+
 int csadv_tile_rows(int32_t tile) {
   if (tile < 0 || tile >= kNumTiles) return CSADV_E_ARG;
   return kSegRows[tile];
@@ -905,13 +928,15 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
   h->dtype = solves[0].dtype;
   h->recon = solves[0].recon;
   h->rk2 = solves[0].dp == 2;
+  h->general = solves[0].et != 3 || solves[0].mf == 2;
   for (int s = 0; s < nsolve; ++s) {
     std::string msg;
     int rc = validate(solves[s], msg);
     if (rc == CSADV_OK && (solves[s].dtype != h->dtype || solves[s].recon != h->recon ||
-                           solves[s].dp != solves[0].dp)) {
+                           solves[s].dp != solves[0].dp ||
+                           (solves[s].et != 3 || solves[s].mf == 2) != h->general)) {
       rc = CSADV_E_ARG;
-      msg = "all solves of one handle share dtype, recon and dp";
+      msg = "all solves of one handle share dtype, recon, dp and the step kind (fused or general)";
     }
     if (rc != CSADV_OK) {
       // the handle is returned so that csadv_last_error() can explain the failure
@@ -1027,6 +1052,14 @@ int csadv_wind_init(csadv_t *h, void *stream) {
   HIPCHK(h, hipMemcpyAsync(h->wsets, sets.data(), sizeof(FieldSet) * ns, hipMemcpyHostToDevice, st));
   for (int s = 0; s < ns; ++s) {
     const csadv_solve_desc &d = h->desc[s];
+    if (d.et != 3) {
+      // ET-S72 / ET-PL07 have no duo-grid wind halo; RK2 copies the neighbours' winds next to
+      // each side (src/edges_treatment.py:296-347)
+      if (d.dp == 2)
+        hipLaunchKernelGGL(gp::k_gp_wind_rk2_copy, dim3(gp_blocks((int64_t)d.nbasis * 24 * d.N)), dim3(BLOCK), 0, st,
+                           h->dev, s);
+      continue;
+    }
     const int blocks = std::min(1024, (d.N * d.N + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL(k_wind_e2c, dim3(blocks, 6, d.nbasis), dim3(BLOCK), 0, st, h->dev, s);
     hipLaunchKernelGGL(k_ghost_edges<double>, dim3(96, 1), dim3(BLOCK), 0, st, h->wsets + s);
@@ -1050,8 +1083,95 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
 
 namespace {
 
+template <typename T, int R>
+void gp_recon(csadv_t *h, hipStream_t st, int s, int tr, int dir, int field) {
+  const csadv_solve_desc &d = h->desc[s];
+  const int64_t total = (int64_t)6 * (d.N + 2 * NG) * (d.N + 2);
+  hipLaunchKernelGGL((gp::k_gp_recon<T, R>), dim3(gp_blocks(total)), dim3(BLOCK), 0, st, h->dev, s, tr, dir, field);
+}
+
+template <typename T, int R>
+void gp_edges(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
+  const csadv_solve_desc &d = h->desc[s];
+  const int64_t sides = (int64_t)6 * 4 * d.N;
+  hipLaunchKernelGGL((gp::k_gp_extrap<T, R>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
+                     fy_field);
+  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 0);
+  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 1);
+  hipLaunchKernelGGL((gp::k_gp_edge_ghosts<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
+}
+
+template <typename T>
+void gp_copy_halo(csadv_t *h, hipStream_t st, int s, int tr, int mode) {
+  const int M = h->desc[s].N + 2 * NG;
+  hipLaunchKernelGGL((gp::k_gp_copy_gather<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
+                     mode);
+  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)48 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
+                     mode, 0);
+  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)48 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
+                     mode, 1);
+}
+
+// One step of every solve by the general step (src/discrete_operators.py:18-101 stage by stage).
+template <typename T, int R>
+int enqueue_general_r(csadv_t *h, hipStream_t st) {
+  const int ns = (int)h->desc.size();
+  bool any_dg = false;
+  for (auto &d : h->desc) any_dg = any_dg || d.et == 3;
+  if (any_dg) {                  // ET-DG solves: the full duo-grid halo of Q
+    const int rc = ghost_fill(h, st);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < ns; ++s) {
+    const csadv_solve_desc &d = h->desc[s];
+    const int N = d.N, M = d.N + 2 * NG;
+    const int64_t edges = (int64_t)6 * M * (N + 1), cells = (int64_t)6 * M * M;
+    for (int tr = 0; tr < d.ntr; ++tr) {
+      if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 0);
+      gp_recon<T, R>(h, st, s, tr, 0, 0);
+      gp_recon<T, R>(h, st, s, tr, 1, 0);
+      if (d.et == 2) gp_edges<T, R>(h, st, s, tr, 0, 0);
+      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 0, 0);
+      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 1, 0);
+      hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
+      // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
+      gp_recon<T, R>(h, st, s, tr, 0, 2);
+      gp_recon<T, R>(h, st, s, tr, 1, 1);
+      if (d.et == 2) gp_edges<T, R>(h, st, s, tr, 2, 1);
+      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 0, 2);
+      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 1, 1);
+      if (d.mf == 2) {
+        const int64_t sides = (int64_t)6 * 4 * N;
+        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 0);
+        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 1);
+      }
+      hipLaunchKernelGGL((gp::k_gp_div<T>), dim3(gp::kRedGen), dim3(BLOCK), 0, st, h->dev, s, tr);
+      hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+    }
+    hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);
+  }
+  HIPCHK(h, hipGetLastError());
+  return CSADV_OK;
+}
+
+template <typename T>
+int enqueue_general_t(csadv_t *h, hipStream_t st) {
+  switch (h->recon) {
+    case 1: return enqueue_general_r<T, 1>(h, st);
+    case 2: return enqueue_general_r<T, 2>(h, st);
+    case 4: return enqueue_general_r<T, 4>(h, st);
+    default: return enqueue_general_r<T, 3>(h, st);
+  }
+}
+
 // The kernels of one step, in order, on stream st (event pairs only when armed).
 int enqueue_step(csadv_t *h, hipStream_t st, bool allow_prof) {
+  if (h->general) {
+    const int rc = wind_avg(h, st, false);
+    if (rc) return rc;
+    return h->dtype == CSADV_F64 ? enqueue_general_t<double>(h, st) : enqueue_general_t<float>(h, st);
+  }
   int rc = wind_avg(h, st, false);
   if (rc) return rc;
   rc = step_ghosts(h, st);

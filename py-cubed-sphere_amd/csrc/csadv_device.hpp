@@ -48,14 +48,14 @@ __device__ __forceinline__ void halo_src(const int32_t *o, int d, int r, int &q,
 }
 
 // Ghost target of (side, depth d, along-edge k); sides 0 E, 1 W, 2 N, 3 S.
+// Written with selects, not a switch: where the side differs across a wavefront, hipcc
+// (ROCm 7.2, gfx950) lowered the four-way switch so that the default arm's lanes kept a
+// stale register for one of the two results.
 __device__ __forceinline__ void ghost_target(int side, int d, int k, int N, int &i, int &j) {
-  const int iend = N + NG;
-  switch (side) {
-    case 0: i = iend + d; j = k; break;
-    case 1: i = NG - 1 - d; j = k; break;
-    case 2: i = k; j = iend + d; break;
-    default: i = k; j = NG - 1 - d; break;
-  }
+  const int g = (side == 0 || side == 2) ? N + NG + d : NG - 1 - d;   // index across the side
+  const bool xside = side < 2;
+  i = xside ? g : k;
+  j = xside ? k : g;
 }
 
 template <typename T> __device__ __forceinline__ T sgn(T x) { return (x > T(0)) ? T(1) : ((x < T(0)) ? T(-1) : T(0)); }

@@ -1,0 +1,542 @@
+// General step: the options the fused step kernel does not cover -- ET-S72 / ET-PL07
+// (adjacent-panel copy halo, PL07 edge extrapolation with cube-edge averaging) and MF-AF
+// (cube-edge flux averaging) -- as a sequence of simple kernels over whole arrays, in the
+// order of src/discrete_operators.py:18-101:
+//
+//   copy halo of Q (ET-S72/PL07; ET-DG uses the duo-grid fill)   src/interpolation.py:320-340
+//   x / y reconstruction of Q                                    src/reconstruction_1d.py:26-384
+//   [ET-PL07] edge extrapolation, cube-edge averaging, ghosts    src/edges_treatment.py:82-178
+//   x / y upwind fluxes                                          src/flux.py:20-128
+//   F / G operators and the split: Qx, Qy                        src/discrete_operators.py:31-73
+//   [ET-S72/PL07] copy halo of Qx / Qy                           src/discrete_operators.py:76-78
+//   second pass on (Qy, Qx), [MF-AF] flux averaging              src/discrete_operators.py:81-86
+//   divergence, [MF-PR] projection, update                       src/discrete_operators.py:95-101
+//
+// Every cross-panel step reads the halo orientation table: the neighbour cell next to the
+// ghost of (panel p, side s) at depth 0 is the neighbour's boundary cell; the direction in
+// which depth grows names the neighbour's side, and the product of our outward and its
+// inward axis sign says whether edge values swap (q_L <-> q_R) and fluxes change sign.
+// MF-PR is applied in the step itself here (no pending correction), as the reference does.
+//
+// Included by csadv.hip (same translation unit); kernels take the solve and the tracer.
+
+namespace gp {
+
+// ---------------------------------------------------------------- work layout
+// Element counts of one solve's work buffer (dtype T), one tracer at a time.
+struct Work {
+  int64_t cell, xe, ye, strips;          // element counts
+  int64_t o_qx, o_qy, o_pxl, o_pxr, o_pyl, o_pyr, o_fx, o_fy, o_div, o_strip, o_avg, o_red;
+  int64_t total;                         // elements of T (+ the FP64 reduction tail)
+};
+
+__host__ __device__ inline Work layout(int N, int ld, int ldu) {
+  Work w{};
+  const int M = N + 2 * NG;
+  w.cell = (int64_t)6 * M * ld;
+  w.xe = (int64_t)6 * M * ldu;
+  w.ye = (int64_t)6 * (M + 1) * ld;
+  w.strips = (int64_t)6 * 4 * 4 * M;
+  int64_t o = 0;
+  w.o_qx = o; o += w.cell;
+  w.o_qy = o; o += w.cell;
+  w.o_pxl = o; o += w.cell;
+  w.o_pxr = o; o += w.cell;
+  w.o_pyl = o; o += w.cell;
+  w.o_pyr = o; o += w.cell;
+  w.o_fx = o; o += w.xe;
+  w.o_fy = o; o += w.ye;
+  w.o_div = o; o += w.cell;
+  w.o_strip = o; o += 2 * w.strips;      // copy-halo strips (two fields)
+  w.o_avg = o; o += (int64_t)6 * 4 * M;  // cube-edge averages
+  o = (o + 7) / 8 * 8;
+  w.o_red = o;                           // FP64 partial sums, kRedGen doubles (in T units)
+  w.total = o;
+  return w;
+}
+constexpr int kRedGen = 512;             // blocks of the divergence reduction
+
+template <typename T>
+struct View {
+  const Solve *S;
+  T *qin, *qout, *qx, *qy, *pxl, *pxr, *pyl, *pyr, *fx, *fy, *div, *strip, *avg;
+  double *red;
+  int N, M, ld, ldu, i0, iend;
+  int64_t pstride, pstride_u, pstride_v, cell;
+};
+
+// The four edge-value arrays by slot: 0 x q_L, 1 x q_R, 2 y q_L, 3 y q_R.  They are
+// consecutive in the work buffer, so a slot computed per lane picks one without a branch
+// (a per-lane multiway choice between arrays is kept out of these kernels: hipcc for gfx950
+// mis-lowered a divergent switch of that kind, leaving one arm's result unset).
+template <typename T>
+__device__ __forceinline__ T *parab(const View<T> &v, int slot) { return v.pxl + slot * v.cell; }
+
+template <typename T>
+__device__ __forceinline__ View<T> view(const Solve *solves, int s, int tr) {
+  const Solve &S = solves[s];
+  const csadv_solve_desc &D = S.d;
+  View<T> v;
+  v.S = &S;
+  v.N = D.N; v.M = S.M; v.ld = D.ld; v.ldu = D.ldu;
+  v.i0 = NG; v.iend = D.N + NG;
+  v.pstride = S.pstride; v.pstride_u = S.pstride_u; v.pstride_v = S.pstride_v;
+  const int steps = D.ctl[1];
+  v.cell = layout(D.N, D.ld, D.ldu).cell;
+  T *q0 = reinterpret_cast<T *>(D.q0) + tr * S.tstride, *q1 = reinterpret_cast<T *>(D.q1) + tr * S.tstride;
+  v.qin = (steps & 1) ? q1 : q0;
+  v.qout = (steps & 1) ? q0 : q1;
+  const Work w = layout(D.N, D.ld, D.ldu);
+  T *b = reinterpret_cast<T *>(D.work);
+  v.qx = b + w.o_qx; v.qy = b + w.o_qy;
+  v.pxl = b + w.o_pxl; v.pxr = b + w.o_pxr; v.pyl = b + w.o_pyl; v.pyr = b + w.o_pyr;
+  v.fx = b + w.o_fx; v.fy = b + w.o_fy; v.div = b + w.o_div;
+  v.strip = b + w.o_strip; v.avg = b + w.o_avg;
+  v.red = reinterpret_cast<double *>(b + w.o_red);
+  return v;
+}
+
+__device__ __forceinline__ bool finished(const csadv_solve_desc &D) { return D.stop > 0 && D.ctl[1] >= D.stop; }
+
+// Side of the neighbour across (p, side) and the sign relating the two axes across the edge.
+// o = orient row {q, ib, id, ir, jb, jd, jr}; sides 0 E, 1 W, 2 N, 3 S.
+__device__ __forceinline__ void across(const int32_t *o, int side, int &sq, int &sign) {
+  const int id = o[2], jd = o[5];
+  sq = (id > 0) + (id == 0) * (2 + (jd > 0));       // id > 0: W; id < 0: E; id == 0: jd > 0 S, else N
+  const int out = (side == 0 || side == 2) ? 1 : -1;
+  const int in = (sq == 1 || sq == 3) ? 1 : -1;
+  sign = out * in;
+}
+
+// cell (i, j) of panel p in a cell array
+__device__ __forceinline__ int64_t cidx(const int64_t ps, int ld, int p, int j, int i) { return p * ps + (int64_t)j * ld + i; }
+
+// ---------------------------------------------------------------- copy halo
+// Gather: every ghost (p, side, depth d, along r) of the target fields reads the neighbour
+// cell named by the orientation table, from field A or B; all reads before any write.
+//   mode 0: the step's Q halo (both from Q);
+//   mode 1: between the passes: E/W ghosts of Qy read Qx for panels 4, 5 and Qy otherwise;
+//           N/S ghosts of Qx read Qy for panels 1, 3 and Qx otherwise (src/halo_data.py:191-400).
+template <typename T>
+__global__ void k_gp_copy_gather(const Solve *solves, int s, int tr, int mode) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned M = v.M, total = 96u * M;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned r = t % M, u = t / M;              // u = (p * 4 + side) * 4 + d
+    const int d = u & 3, side = (u >> 2) & 3, p = u >> 4;
+    int q, i, j;
+    halo_src(D.orient + (p * 4 + side) * 7, d, r, q, i, j);
+    const T *ew = (p == 4 || p == 5) ? v.qx : v.qy;
+    const T *ns = (p == 1 || p == 3) ? v.qy : v.qx;
+    const T *src = mode == 0 ? v.qin : (side < 2 ? ew : ns);
+    v.strip[t] = src[cidx(v.pstride, v.ld, q, j, i)];
+  }
+}
+
+// Scatter of one side group (E/W, or N/S) into its target field: E/W ghosts go to Qy (or Q),
+// N/S ghosts to Qx (or Q).  The N/S group is launched after the E/W group, so where the
+// two overlap in one field (mode 0) the N/S values are the ones that stay, as in the
+// reference's order.
+template <typename T>
+__global__ void k_gp_copy_scatter(const Solve *solves, int s, int tr, int mode, int ns_group) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned M = v.M, total = 48u * M;
+  T *dst = mode == 0 ? v.qin : (ns_group ? v.qx : v.qy);
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned r = t % M, u = t / M;              // u = (p * 2 + side - group) * 4 + d
+    const int d = u & 3, side = ((u >> 2) & 1) + (ns_group ? 2 : 0), p = u >> 3;
+    int ti, tj;
+    ghost_target(side, d, r, v.N, ti, tj);
+    dst[cidx(v.pstride, v.ld, p, tj, ti)] = v.strip[((p * 4 + side) * 4 + d) * M + r];
+  }
+}
+
+// ---------------------------------------------------------------- reconstruction
+// Edge values of cells i0-1 .. iend along dir (0 = x: every row; 1 = y: every column) of
+// field F (0 = Q, 1 = Qx, 2 = Qy); src/reconstruction_1d.py:26-384.
+template <typename T, int RECON>
+__global__ void k_gp_recon(const Solve *solves, int s, int tr, int dir, int field) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const T *F = field == 0 ? v.qin : (field == 1 ? v.qx : v.qy);
+  T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
+  const unsigned M = v.M, nc = v.N + 2, total = 6u * M * nc;
+  const int st = dir == 0 ? 1 : v.ld;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / nc;
+    const int c = (int)(t % nc) + v.i0 - 1;          // cell along dir
+    const int a = u % M;                             // the other index
+    const int p = u / M;
+    const int64_t o = dir == 0 ? cidx(v.pstride, v.ld, p, a, c) : cidx(v.pstride, v.ld, p, c, a);
+    T qL, qR;
+    recon_edges<T, RECON>(F[o - 2 * st], F[o - st], F[o], F[o + st], F[o + 2 * st], qL, qR);
+    L[o] = qL;
+    R[o] = qR;
+  }
+}
+
+// PL07 edge extrapolation at the four panel sides (eqs. 47 and 49;
+// src/edges_treatment.py:82-114): one thread per (panel, side, interior position).
+template <typename T, int RECON>
+__global__ void k_gp_extrap(const Solve *solves, int s, int tr, int fx_field, int fy_field) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const T *Fx = fx_field == 0 ? v.qin : (fx_field == 1 ? v.qx : v.qy);
+  const T *Fy = fy_field == 0 ? v.qin : (fy_field == 1 ? v.qx : v.qy);
+  const unsigned N = v.N, total = 24u * N;
+  const int i0 = v.i0, iend = v.iend;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;
+    const int k = (int)(t % N) + i0;
+    const int side = u & 3, p = u >> 2;
+    const bool xdir = side < 2, low = side & 1;   // W and S are the low ends (cells i0, i0+1)
+    const T *F = xdir ? Fx : Fy;
+    const int st = xdir ? 1 : v.ld;
+    const int dir = low ? st : -st;               // from the boundary cell into the panel
+    const int64_t c0 = (xdir ? cidx(v.pstride, v.ld, p, k, 0) : cidx(v.pstride, v.ld, p, 0, k)) +
+                       (int64_t)(low ? i0 : iend - 1) * st;
+    // A: the edge on the panel side (q_L at the low end, q_R at the high end); B: the other
+    T *A = parab(v, (xdir ? 0 : 2) + (low ? 0 : 1)), *B = parab(v, (xdir ? 0 : 2) + (low ? 1 : 0));
+    const T Q0 = F[c0], Q1 = F[c0 + dir], Q2 = F[c0 + 2 * dir];
+    A[c0] = T(1.5) * Q0 - T(0.5) * Q1;
+    const T e = (T(3.0) * Q0 + T(11.0) * Q1 - T(2.0) * (Q2 - Q0)) / T(14.0);
+    B[c0] = e;
+    A[c0 + dir] = e;
+    if (RECON == 3) B[c0 + dir] = A[c0 + 2 * dir];
+  }
+}
+
+// The edge value of the boundary cell of (p, side) facing the cube edge, along position k:
+// E: q_R of cell iend-1 (x), W: q_L of cell i0 (x), N: q_R of cell jend-1 (y), S: q_L of j0 (y).
+// The slot of that edge is side ^ 1 (E 1, W 0, N 3, S 2).
+template <typename T>
+__device__ __forceinline__ int64_t boundary_cell(const View<T> &v, int p, int side, int k) {
+  const bool xdir = side < 2, high = side == 0 || side == 2;
+  const int b = high ? v.iend - 1 : v.i0;
+  return xdir ? cidx(v.pstride, v.ld, p, k, b) : cidx(v.pstride, v.ld, p, b, k);
+}
+
+// Cube-edge averaging of the facing edge values (src/edges_treatment.py:31-76): gather the
+// averages of all 24 sides, then write them (two launches: the values read are written).
+template <typename T>
+__global__ void k_gp_edge_avg(const Solve *solves, int s, int tr, int write) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned N = v.N, total = 24u * N;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;
+    const int k = (int)(t % N) + v.i0;
+    const int side = u & 3, p = u >> 2;
+    T *own = parab(v, side ^ 1);
+    const int64_t oi = boundary_cell(v, p, side, k);
+    if (write) {
+      own[oi] = v.avg[t];
+      continue;
+    }
+    const int32_t *o = D.orient + (p * 4 + side) * 7;
+    int q, i, j, sq, sign;
+    halo_src(o, 0, k, q, i, j);
+    across(o, side, sq, sign);
+    const T *nb = parab(v, sq ^ 1);
+    v.avg[t] = (own[oi] + nb[cidx(v.pstride, v.ld, q, j, i)]) * T(0.5);
+  }
+}
+
+// Ghost-cell parabolas (src/edges_treatment.py:119-178): the ghost cell next to (p, side)
+// takes the neighbour's boundary-cell edge values, swapped when the axes point opposite ways.
+template <typename T>
+__global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned N = v.N, total = 24u * N;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;
+    const int k = (int)(t % N) + v.i0;
+    const int side = u & 3, p = u >> 2;
+    const int32_t *o = D.orient + (p * 4 + side) * 7;
+    int q, i, j, sq, sign;
+    halo_src(o, 0, k, q, i, j);
+    across(o, side, sq, sign);
+    const int nslot = sq < 2 ? 0 : 2;                // the neighbour's x or y edge values
+    const int64_t ni = cidx(v.pstride, v.ld, q, j, i);
+    const T l = parab(v, nslot)[ni], r = parab(v, nslot + 1)[ni];
+    const bool xdir = side < 2;
+    const int g = (side == 0 || side == 2) ? v.iend : v.i0 - 1;
+    const int64_t gi = xdir ? cidx(v.pstride, v.ld, p, k, g) : cidx(v.pstride, v.ld, p, g, k);
+    T *L = parab(v, xdir ? 0 : 2), *R = parab(v, xdir ? 1 : 3);
+    L[gi] = sign > 0 ? l : r;
+    R[gi] = sign > 0 ? r : l;
+  }
+}
+
+// ---------------------------------------------------------------- fluxes
+// Upwind PPM flux at x-edges i0..iend of every row (dir 0) or y-edges j0..jend of every
+// column (dir 1) from the reconstruction of field F (src/flux.py:20-128).
+template <typename T>
+__global__ void k_gp_flux(const Solve *solves, int s, int tr, int dir, int field) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const T *F = field == 0 ? v.qin : (field == 1 ? v.qx : v.qy);
+  const unsigned M = v.M, ne = v.N + 1, total = 6u * M * ne;
+  const bool mt0 = D.mt == 1;
+  const T *gc = reinterpret_cast<const T *>(D.gc);
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / ne;
+    const int e = (int)(t % ne) + v.i0;
+    const int a = u % M;
+    const int p = u / M;
+    T uav, usg, ge_l, ge_r, gce, ge;
+    int64_t cl, cr, fo;
+    const T *ge_arr;
+    int ge_st;
+    int64_t ge_base;
+    if (dir == 0) {
+      const T *U = reinterpret_cast<const T *>(D.uavg) + p * v.pstride_u;
+      const T *US = D.usgn ? reinterpret_cast<const T *>(D.usgn) + p * v.pstride_u : U;
+      uav = U[(int64_t)a * v.ldu + e];
+      usg = US[(int64_t)a * v.ldu + e];
+      cl = cidx(v.pstride, v.ld, p, a, e - 1);
+      cr = cl + 1;
+      ge_arr = reinterpret_cast<const T *>(D.gu);
+      ge_base = (int64_t)a * v.ldu;
+      ge_st = 1;
+      fo = p * v.pstride_u + (int64_t)a * v.ldu + e;
+    } else {
+      const T *V = reinterpret_cast<const T *>(D.vavg) + p * v.pstride_v;
+      const T *VS = D.vsgn ? reinterpret_cast<const T *>(D.vsgn) + p * v.pstride_v : V;
+      uav = V[(int64_t)e * v.ld + a];
+      usg = VS[(int64_t)e * v.ld + a];
+      cl = cidx(v.pstride, v.ld, p, e - 1, a);
+      cr = cl + v.ld;
+      ge_arr = reinterpret_cast<const T *>(D.gv);
+      ge_base = a;
+      ge_st = v.ld;
+      fo = p * v.pstride_v + (int64_t)e * v.ld + a;
+    }
+    const bool up = usg >= T(0);
+    const int64_t c = up ? cl : cr;
+    const int ec = up ? e - 1 : e;             // upwind cell index along dir
+    const T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
+    T qL = L[c], qR = R[c], q = F[c];
+    ge_l = ge_arr[ge_base + (int64_t)ec * ge_st];
+    ge_r = ge_arr[ge_base + (int64_t)(ec + 1) * ge_st];
+    gce = dir == 0 ? gc[(int64_t)a * v.ld + ec] : gc[(int64_t)ec * v.ld + a];
+    ge = ge_arr[ge_base + (int64_t)e * ge_st];
+    if (mt0) { qL = qL * ge_l; qR = qR * ge_r; q = q * gce; }
+    const T dq = qR - qL;
+    const T q6 = T(6) * q - T(3) * (qR + qL);
+    const T cc = (uav * T(D.dt)) / T(dir == 0 ? D.dx : D.dy);
+    T f = up ? (qR + cc * T(0.5) * (q6 - dq) - q6 * cc * cc / T(3.0))
+             : (qL - cc * T(0.5) * (q6 + dq) - q6 * cc * cc / T(3.0));
+    f = f * uav;
+    if (!mt0) f = f * ge;
+    (dir == 0 ? v.fx : v.fy)[fo] = f;
+  }
+}
+
+// MF-AF (src/edges_treatment.py:231-278): each cube-edge flux becomes 0.5 f + 0.5 (sign f'),
+// f' the neighbour's flux on the same edge; gather all 24 sides, then write.
+template <typename T>
+__global__ void k_gp_flux_avg(const Solve *solves, int s, int tr, int write) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned N = v.N, total = 24u * N;
+  // flux on the edge of side (p, side) at position a: a row of the x-edge array (E, W) or a
+  // column of the y-edge array (N, S); both offsets are formed and one is selected
+  auto fidx = [&](int p, int side, int a) -> T * {
+    const int e = (side == 0 || side == 2) ? v.iend : v.i0;
+    const int64_t ox = p * v.pstride_u + (int64_t)a * v.ldu + e;
+    const int64_t oy = p * v.pstride_v + (int64_t)e * v.ld + a;
+    return side < 2 ? v.fx + ox : v.fy + oy;
+  };
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;
+    const int k = (int)(t % N) + v.i0;
+    const int side = u & 3, p = u >> 2;
+    T *own = fidx(p, side, k);
+    if (write) {
+      *own = v.avg[t];
+      continue;
+    }
+    const int32_t *o = D.orient + (p * 4 + side) * 7;
+    int q, i, j, sq, sign;
+    halo_src(o, 0, k, q, i, j);
+    across(o, side, sq, sign);
+    const T nb = *fidx(q, sq, sq < 2 ? j : i);
+    v.avg[t] = T(0.5) * *own + T(0.5) * (sign > 0 ? nb : -nb);
+  }
+}
+
+// ---------------------------------------------------------------- split, divergence, update
+// dF of cell i (x) / j (y) from the fluxes: -(f[c+1] - f[c]) * dt / dx inside [i0, iend), else 0
+// (the edges read exist for every cell of the panel: i + 1 <= M < ldu, j + 1 <= M rows of M + 1)
+template <typename T>
+__device__ __forceinline__ T dflux_x(const View<T> &v, const csadv_solve_desc &D, int p, int j, int i) {
+  const T *f = v.fx + p * v.pstride_u + (int64_t)j * v.ldu;
+  const T d = -(f[i + 1] - f[i]);
+  const T r = d * T(D.dt) / T(D.dx);
+  return (i >= v.i0 && i < v.iend) ? r : T(0);
+}
+template <typename T>
+__device__ __forceinline__ T dflux_y(const View<T> &v, const csadv_solve_desc &D, int p, int j, int i) {
+  const T *f = v.fy + p * v.pstride_v + i;
+  const T d = -(f[(int64_t)(j + 1) * v.ld] - f[(int64_t)j * v.ld]);
+  const T r = d * T(D.dt) / T(D.dy);
+  return (j >= v.i0 && j < v.iend) ? r : T(0);
+}
+
+// Qx, Qy of every cell (src/discrete_operators.py:31-73)
+template <typename T>
+__global__ void k_gp_split(const Solve *solves, int s, int tr) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned M = v.M, total = 6u * M * M;
+  const T *gc = reinterpret_cast<const T *>(D.gc), *gu = reinterpret_cast<const T *>(D.gu),
+          *gv = reinterpret_cast<const T *>(D.gv);
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / M;
+    const int i = t % M, j = u % M, p = u / M;
+    const int64_t c = cidx(v.pstride, v.ld, p, j, i);
+    const T Q = v.qin[c], g = gc[(int64_t)j * v.ld + i];
+    const T dFx = dflux_x(v, D, p, j, i), dFy = dflux_y(v, D, p, j, i);
+    T Qx, Qy;
+    if (D.split == 1) {
+      const T gQ = Q * g;
+      Qx = (gQ + T(0.5) * dFx) / g;
+      Qy = (gQ + T(0.5) * dFy) / g;
+    } else {
+      const T *U = reinterpret_cast<const T *>(D.uavg) + p * v.pstride_u;
+      const T *V = reinterpret_cast<const T *>(D.vavg) + p * v.pstride_v;
+      const T dt = T(D.dt), dx = T(D.dx), dy = T(D.dy);     // cx = u dt / dx (src/cfl.py:10-19)
+      const T cx1 = (U[(int64_t)j * v.ldu + i + 1] * dt) / dx, cx2 = (U[(int64_t)j * v.ldu + i] * dt) / dx;
+      const T cy1 = (V[(int64_t)(j + 1) * v.ld + i] * dt) / dy, cy2 = (V[(int64_t)j * v.ld + i] * dt) / dy;
+      const T c1x = gu[(int64_t)j * v.ldu + i + 1] * cx1, c2x = gu[(int64_t)j * v.ldu + i] * cx2;
+      const T c1y = gv[(int64_t)(j + 1) * v.ld + i] * cy1, c2y = gv[(int64_t)j * v.ld + i] * cy2;
+      if (D.split == 2) {
+        const T gQ = Q * g;
+        Qx = (gQ + T(0.5) * dFx + T(0.5) * (c1x - c2x) * Q) / g;
+        Qy = (gQ + T(0.5) * dFy + T(0.5) * (c1y - c2y) * Q) / g;
+      } else {
+        Qx = T(0.5) * (Q + (Q + dFx) / (T(1.0) - (c1x - c2x)));
+        Qy = T(0.5) * (Q + (Q + dFy) / (T(1.0) - (c1y - c2y)));
+      }
+    }
+    v.qx[c] = Qx;
+    v.qy[c] = Qy;
+  }
+}
+
+// div = -(dF + dG) / (dt g) on the interior and, for MF-PR, per-block FP64 sums of div g
+template <typename T>
+__global__ void k_gp_div(const Solve *solves, int s, int tr) {
+  __shared__ double red[BLOCK];
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned N = v.N, total = 6u * N * N;
+  const T *gc = reinterpret_cast<const T *>(D.gc);
+  double acc = 0.0;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;
+    const int i = (int)(t % N) + v.i0, j = (int)(u % N) + v.i0, p = u / N;
+    const T g = gc[(int64_t)j * v.ld + i];
+    const T dv = -(dflux_x(v, D, p, j, i) + dflux_y(v, D, p, j, i)) / (T(D.dt) * g);
+    v.div[cidx(v.pstride, v.ld, p, j, i)] = dv;
+    acc += (double)dv * (double)g;
+  }
+  if (D.mf != 3) return;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) v.red[blockIdx.x] = red[0];
+}
+
+// Q^{n+1} = Q^n - dt div on the interior (MF-PR: div - g m0 / a2), every other cell copied
+// so that the ghost cells persist across steps as in the reference's single array.
+template <typename T>
+__global__ void k_gp_update(const Solve *solves, int s, int tr) {
+  __shared__ double red[BLOCK];
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  double m0 = 0.0;
+  if (D.mf == 3) {
+    red[threadIdx.x] = 0.0;
+    for (int b = threadIdx.x; b < kRedGen; b += BLOCK) red[threadIdx.x] += v.red[b];
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    m0 = red[0];
+  }
+  const unsigned M = v.M, total = 6u * M * M;
+  const T *gc = reinterpret_cast<const T *>(D.gc);
+  const bool pr = D.mf == 3;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / M;
+    const int i = t % M, j = u % M, p = u / M;
+    const int64_t c = cidx(v.pstride, v.ld, p, j, i);
+    const bool interior = i >= v.i0 && i < v.iend && j >= v.i0 && j < v.iend;
+    const T q = v.qin[c];
+    T dv = v.div[c];                       // every cell of div exists; only interior ones are used
+    if (pr) dv = dv - (gc[(int64_t)j * v.ld + i] * T(m0)) / T(D.a2);
+    v.qout[c] = interior ? q - T(D.dt) * dv : q;
+  }
+}
+
+// ET-S72 / ET-PL07 wind halo for RK2 (src/edges_treatment.py:305-347), on every wind basis
+// field: the ghost edge one outside each panel side takes the neighbour's normal wind at its
+// edge one inside the shared edge, times the axis sign across the edge.
+__global__ void k_gp_wind_rk2_copy(const Solve *solves, int s) {
+  const Solve &S = solves[s];
+  const csadv_solve_desc &D = S.d;
+  const int ld = D.ld, ldu = D.ldu, i0 = NG, iend = D.N + NG;
+  const int64_t bu = 6 * S.pstride_u, bv = 6 * S.pstride_v;
+  const unsigned N = D.N, total = (unsigned)D.nbasis * 24u * N;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;                        // u = (b * 6 + p) * 4 + side
+    const int k = (int)(t % N) + i0;
+    const int side = u & 3, p = (u >> 2) % 6, b = (u >> 2) / 6;
+    double *U = D.wbu + b * bu, *V = D.wbv + b * bv;
+    const int32_t *o = D.orient + (p * 4 + side) * 7;
+    int q, i, j, sq, sign;
+    halo_src(o, 0, k, q, i, j);
+    across(o, side, sq, sign);
+    // the neighbour's normal wind one edge inside the shared edge, x-edges (E/W) or y-edges (N/S);
+    // both addresses are formed and one is selected
+    const int ne = (sq == 0 || sq == 2) ? iend - 1 : i0 + 1;
+    const double *nu = U + q * S.pstride_u + (int64_t)j * ldu + ne;
+    const double *nv = V + q * S.pstride_v + (int64_t)ne * ld + i;
+    const double nb = *(sq < 2 ? nu : nv);
+    const double val = sign > 0 ? nb : -nb;
+    // our ghost edge one outside the side
+    const int te = (side == 0 || side == 2) ? iend + 1 : i0 - 1;
+    double *tu = U + p * S.pstride_u + (int64_t)k * ldu + te;
+    double *tv = V + p * S.pstride_v + (int64_t)te * ld + k;
+    *(side < 2 ? tu : tv) = val;
+  }
+}
+
+// the step is complete for every tracer: advance the step counter
+__global__ void k_gp_advance(const Solve *solves, int s) {
+  const csadv_solve_desc &D = solves[s].d;
+  if (D.stop > 0 && D.ctl[1] >= D.stop) return;
+  if (threadIdx.x == 0) D.ctl[1] = D.ctl[1] + 1;
+}
+
+}  // namespace gp

@@ -10,12 +10,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcsadv.so")
 
 CSADV_F64, CSADV_F32 = 0, 1
-ABI_VERSION = 2          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
+ABI_VERSION = 3          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
 _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of order", -4: "size out of range"}
 
 # exported symbols, in declaration order of include/csadv.h
 SYMBOLS = (
-    "csadv_create", "csadv_partials_per_tracer", "csadv_tile_rows", "csadv_update", "csadv_wind_init",
+    "csadv_create", "csadv_partials_per_tracer", "csadv_work_bytes", "csadv_tile_rows", "csadv_update", "csadv_wind_init",
     "csadv_ghost_fill", "csadv_step", "csadv_set_graph", "csadv_finalize", "csadv_mass", "csadv_norms",
     "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_last_error", "csadv_destroy",
     "csadv_abi_version",
@@ -42,7 +42,7 @@ class SolveDesc(ctypes.Structure):
         ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p), ("strip_gw", _p),
         ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p), ("corner_gw", _p),
         ("coef", _p), ("corr", _p), ("partials", _p), ("ctl", _p),
-        ("tile", ctypes.c_int32), ("stop", ctypes.c_int32),
+        ("tile", ctypes.c_int32), ("stop", ctypes.c_int32), ("work", _p),
     ]
 
 
@@ -65,6 +65,7 @@ def load():
     lib.csadv_create.argtypes = [ctypes.POINTER(SolveDesc), i32, i32, ctypes.POINTER(vp)]
     lib.csadv_partials_per_tracer.argtypes = [i32, i32]
     lib.csadv_tile_rows.argtypes = [i32]
+    lib.csadv_work_bytes.argtypes = [i32, i32, ctypes.POINTER(ctypes.c_int64)]
     lib.csadv_update.argtypes = [vp, ctypes.POINTER(SolveDesc), i32, vp]
     lib.csadv_wind_init.argtypes = [vp, vp]
     lib.csadv_ghost_fill.argtypes = [vp, vp]

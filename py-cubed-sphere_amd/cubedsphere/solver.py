@@ -198,11 +198,19 @@ class Advection:
         _lib.check(nparts, None, "csadv_partials_per_tracer")
         self.partials = torch.zeros(ntr * nparts, **f64)
         self.ctl = torch.zeros(4, device=dev, dtype=torch.int32)
+        # ET-S72, ET-PL07 and MF-AF run the library's general step, which needs a work buffer
+        self.work = None
+        if self.opts["et"] != 3 or self.opts["mf"] == 2:
+            nbytes = ctypes.c_int64()
+            code = _lib.load().csadv_work_bytes(N, _lib.CSADV_F64 if self.dtype == "f64" else _lib.CSADV_F32,
+                                                ctypes.byref(nbytes))
+            _lib.check(code, None, "csadv_work_bytes")
+            self.work = torch.zeros(int(nbytes.value), device=dev, dtype=torch.uint8)
 
     def device_bytes(self):
         """Device memory held by this solve's own buffers (tables shared per N excluded)."""
         ts = [self.uavg, self.vavg, self.usgn, self.vsgn, self.wbu, self.wbv, self.wpc, self.coef, self.corr,
-              self.partials, self.ctl] + list(self.q)
+              self.partials, self.ctl, self.work] + list(self.q)
         return sum(t.numel() * t.element_size() for t in ts if t is not None)
 
     def release_wind_scratch(self):
@@ -236,6 +244,7 @@ class Advection:
         d.coef, d.corr, d.partials, d.ctl = _ptr(self.coef), _ptr(self.corr), _ptr(self.partials), _ptr(self.ctl)
         d.tile = self.tile
         d.stop = self.stop
+        d.work = _ptr(self.work)
         return d
 
     def _stream(self):
