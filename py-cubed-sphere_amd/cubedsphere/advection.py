@@ -8,7 +8,7 @@ src/advection_error.py:21-188, main.py test case 5) with the step on the GPU:
   adv_time_step(cs_grid, sim, k, t)      # src/advection_timestep.py:19  (one step on the device)
   update_adv(cs_grid, sim, t)            # src/advection_timestep.py:48  (no-op: see below)
   adv_sphere(cs_grid, sim) -> (linf, l1, l2)
-  error_analysis_adv(sim) -> errors table of the convergence study (tc = 2)
+  error_analysis_adv(sim) -> errors table of the convergence study (tc = 2), rates as the reference's
   main(pardir)                           # test case 5 from par/configuration.par + advection.par
 
 Differences of contract, all deliberate: ``cs_grid`` is a :class:`cubedsphere.grid.Grid`;
@@ -65,12 +65,14 @@ def update_adv(cs_grid, simulation, t):
 
 def adv_sphere(cs_grid, simulation, divtest_flag=False):
     """src/advection_sphere.py:13-61 without plotting: run Tf/dt steps, return the final
-    (L_inf, L_1, L_2) errors against the exact solution."""
+    (L_inf, L_1, L_2) errors against the exact solution.  Nothing is read between steps
+    (there is no per-step output here), so the steps go to the device in one call, which is
+    the same as Nsteps calls of adv_time_step + update_adv."""
     nsteps = 1 if divtest_flag else int(simulation.Tf / simulation.dt)
     init_vars_adv(cs_grid, simulation)
-    for k in range(1, nsteps + 1):
-        adv_time_step(cs_grid, simulation, k, k * simulation.dt)
-        update_adv(cs_grid, simulation, k * simulation.dt)
+    if nsteps >= 64:
+        simulation.use_graph(16)          # replay 16-step graphs: no per-launch host cost
+    simulation.step(nsteps)
     simulation.mass_change = _mass_change(simulation.total_mass0, simulation.mass())
     return simulation.norms(t=nsteps * simulation.dt)
 
@@ -80,9 +82,31 @@ def _mass_change(m0, m):
     return abs(m0 - m) / abs(m0) if abs(m0) > 1e-10 else abs(m0 - m)
 
 
-def error_analysis_adv(simulation, ntest=6, n0=16, device="cuda:0"):
+STUDY_NAMES = ("PL07-RK1", "PL07-RK1-DG-PR", "AVLT-RK2-DG-AF", "ALVT-RK2-DG-PR")   # src/advection_error.py:170
+
+
+def print_errors_simul(error_linf, error_l1, error_l2, i):
+    """src/errors.py:84-94: the errors of run i and, from the second run on, the ratio to run i-1."""
+    if i > 0:
+        print("Norms          (Linf,    L1,       L2) ")
+        print("Error E_" + str(i) + "    :", "{:.2e}".format(error_linf[i]), "{:.2e}".format(error_l1[i]),
+              "{:.2e}".format(error_l2[i]))
+        print("Ratio E_" + str(i) + "/E_" + str(i - 1) + ":", "{:.2e}".format(error_linf[i - 1] / error_linf[i]),
+              "{:.2e}".format(error_l1[i - 1] / error_l1[i]), "{:.2e}".format(error_l2[i - 1] / error_l2[i]))
+    else:
+        print("Error(Linf, L1, L2) :", "{:.2e}".format(error_linf[i]), "{:.2e}".format(error_l1[i]),
+              "{:.2e}".format(error_l2[i]))
+
+
+def convergence_rates(err):
+    """|log E_i - log E_(i-1)| / log 2 along the resolution axis (src/advection_error.py:163-164)."""
+    return np.abs(np.log(err[1:]) - np.log(err[:-1])) / np.log(2.0)
+
+
+def error_analysis_adv(simulation, ntest=6, n0=16, device="cuda:0", verbose=False, dtype="f64"):
     """src/advection_error.py:21-188 without plots: the 4-scheme x ntest-resolution study of the
-    simulation's (ic, vf); returns (Ns, errors [ntest][4][3] as (linf, l1, l2), rates [ntest-1][4][3])."""
+    simulation's (ic, vf), Tf = 5, N = n0 2^i, dt halving from the vf's first step.  Returns
+    (Ns, errors [ntest][4][3] as (linf, l1, l2), rates [ntest-1][4][3])."""
     vf, ic = simulation.vf, simulation.ic
     Ns = [n0 * 2 ** i for i in range(ntest)]
     dts = [STUDY_DT0[vf] * 0.5 ** i for i in range(ntest)]
@@ -90,11 +114,14 @@ def error_analysis_adv(simulation, ntest=6, n0=16, device="cuda:0"):
     for d, (recon, opsplit, et, mt, dp, mf) in enumerate(STUDY_SCHEMES):
         for i, (N, dt) in enumerate(zip(Ns, dts)):
             g = Grid(N)
-            sim = adv_simulation_par(g, dt, 5.0, ic, vf, 2, recon, dp, opsplit, et, mt, mf, device=device)
+            sim = adv_simulation_par(g, dt, 5.0, ic, vf, 2, recon, dp, opsplit, et, mt, mf, dtype=dtype, device=device)
+            if verbose:
+                print("\nParameters: N=" + str(N) + ", dt=" + str(dt), ", scheme=", STUDY_NAMES[d])
             err[i, d] = adv_sphere(g, sim)
             sim.close()
-    rates = np.log2(err[:-1] / err[1:])
-    return Ns, err, rates
+            if verbose:
+                print_errors_simul(err[:, d, 0], err[:, d, 1], err[:, d, 2], i)
+    return Ns, err, convergence_rates(err)
 
 
 def main(pardir="par"):
@@ -108,5 +135,35 @@ def main(pardir="par"):
     if ap.tc == 1:
         return adv_sphere(g, sim)
     if ap.tc == 2:
-        return error_analysis_adv(sim)
+        sim.close()
+        return error_analysis_adv(sim, verbose=True)
     raise CsadvError("invalid advection test case %r" % ap.tc)
+
+
+def _cli(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m cubedsphere.advection",
+                                 description="Test case 5 from the par files, or the convergence study of one (ic, vf).")
+    ap.add_argument("pardir", nargs="?", default="par", help="directory holding configuration.par and advection.par")
+    ap.add_argument("--study", nargs=2, type=int, metavar=("IC", "VF"),
+                    help="run the 4-scheme convergence study of (ic, vf) without par files")
+    ap.add_argument("--ntest", type=int, default=6, help="resolutions of the study: N = 16 * 2^i, i < ntest")
+    ap.add_argument("--dtype", default="f64", choices=("f64", "f32"))
+    args = ap.parse_args(argv)
+    if args.study:
+        class _P:
+            ic, vf = args.study
+        Ns, err, rates = error_analysis_adv(_P, ntest=args.ntest, verbose=True, dtype=args.dtype)
+        print("\nic=%d vf=%d, Tf=5, %s" % (args.study[0], args.study[1], args.dtype))
+        print("%-16s %6s %10s %10s %10s %6s %6s %6s" % ("scheme", "N", "Linf", "L1", "L2", "r_inf", "r_1", "r_2"))
+        for d, name in enumerate(STUDY_NAMES):
+            for i, N in enumerate(Ns):
+                r = rates[i - 1, d] if i else (np.nan, np.nan, np.nan)
+                print("%-16s %6d %10.3e %10.3e %10.3e %6.2f %6.2f %6.2f" % ((name, N) + tuple(err[i, d]) + tuple(r)))
+        return 0
+    print(main(args.pardir))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_cli())

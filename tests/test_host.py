@@ -401,3 +401,18 @@ def test_orientation_reproduces_adjacent_panel_copy_halo():
         assert np.array_equal(out_x, ref_x)
         if not same:
             assert np.array_equal(out_y, ref_y)
+
+
+def test_study_rates_and_printout_follow_the_reference(capsys):
+    """Rates are |log E_i - log E_(i-1)| / log 2 (src/advection_error.py:163-164): halving the
+    error is rate 1 and growing it fourfold is rate 2, not -2.  The run printout has the layout
+    of src/errors.py:84-94."""
+    from cubedsphere.advection import STUDY_SCHEMES, convergence_rates, print_errors_simul
+    err = np.array([[1.0, 2.0, 4.0], [0.5, 0.5, 16.0]])
+    assert np.allclose(convergence_rates(err), [[1.0, 2.0, 2.0]])
+    assert len(STUDY_SCHEMES) == 4
+    print_errors_simul(err[:, 0], err[:, 1], err[:, 2], 1)
+    out = capsys.readouterr().out.splitlines()
+    assert out == ["Norms          (Linf,    L1,       L2) ",
+                   "Error E_1    : 5.00e-01 5.00e-01 1.60e+01",
+                   "Ratio E_1/E_0: 2.00e+00 4.00e+00 2.50e-01"]
