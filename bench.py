@@ -55,7 +55,7 @@ SERVED_REQUESTS, SERVED_RATE = 200, 12.0      # Poisson stream: count, arrivals 
 BATCH_NS = (32, 48, 96, 192, 384, 512)
 # tile ids from the autotuner for the single-solve workloads (the batched workload tunes
 # itself at start-up over BATCH_TILE_CANDIDATES)
-TUNED_TILE = {("f64", 384): 0, ("f32", 768): 0, ("f64", 48): 1}
+TUNED_TILE = {("f64", 384): 0, ("f32", 768): 3, ("f64", 48): 1}
 BATCH_TILE_CANDIDATES = (0, 2, 6, 8, 9, 10)
 
 

@@ -5,7 +5,7 @@
 //                                     used by csadv_ghost_fill and for the wind basis
 //   k_ghost                           per-step ghost fill of the 3 layers a step reads, from
 //                                     flattened recipes, plus the MF-PR sum of the last step
-//   k_step<T,RECON,RK2>               the step: both directional passes of the split, PPM edges,
+//   k_step<T,RECON,RK2,SPLIT>         the step: both directional passes of the split, PPM edges,
 //                                     upwind fluxes, update and the mass-fixer partial sums of one
 //                                     (solve, panel, tracer, column strip, row segment) per
 //                                     wavefront (src/discrete_operators.py:18-101, src/flux.py,
@@ -129,25 +129,27 @@ __device__ __forceinline__ double block_sum_fixed(double v, double *red) {
 
 // Mass-fixer correction of the last step: c = dt * m0 / a2 with m0 = sum of div g over the
 // interior (src/discrete_operators.py:98-101), summed in a fixed order so that every block
-// (and every run) gets the same bits.
-__device__ double pending_corr(const Solve &S, int tr, double *red) {
+// (and every run) gets the same bits.  pending = ctl[3], read by the caller up front: the
+// partial sums are requested whether or not they are pending, so that no load waits on the
+// flag; every thread issues its eight loads of a batch before adding them in the order
+// i, i+BLOCK, ... (masked past the end, so no thread walks its tail one load at a time).
+__device__ double pending_corr(const Solve &S, int tr, double *red, int pending) {
   const csadv_solve_desc &D = S.d;
   if (D.mf != 3) return 0.0;
-  if (!D.ctl[3]) return D.corr[tr];
+  const double held = D.corr[tr];
   const double *P = D.partials + (int64_t)tr * S.part_per_tracer;
   const int n = S.part_per_tracer;
   double acc = 0.0;
-  int i = threadIdx.x;
-  // eight independent loads in flight per thread; the additions keep the order i, i+BLOCK, ...
-  for (; i + 7 * BLOCK < n; i += 8 * BLOCK) {
+  for (int i = threadIdx.x; i < n; i += 8 * BLOCK) {
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = P[i + u * BLOCK];
+    for (int u = 0; u < 8; ++u) v[u] = i + u * BLOCK < n ? P[i + u * BLOCK] : 0.0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += v[u];
+    for (int u = 0; u < 8; ++u)
+      if (i + u * BLOCK < n) acc += v[u];
   }
-  for (; i < n; i += BLOCK) acc += P[i];
-  return D.dt * block_sum_fixed(acc, red) / D.a2;
+  const double sum = block_sum_fixed(acc, red);
+  return pending ? D.dt * sum / D.a2 : held;
 }
 
 template <typename T>
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
       gw = D.corner_gw[c];
     }
   }
-  const int steps = D.ctl[1];
+  const int steps = D.ctl[1], pending = D.ctl[3];
   if (D.stop > 0 && steps >= D.stop) {      // solve finished: k_step reads ctl[2] and skips it too
     if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
     return;
@@ -218,8 +220,8 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
         for (int l = 0; l < 16; ++l) acc += (double)v[l] * w16[l];
       }
     }
-    const double cf = pending_corr(S, tr, red);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && D.ctl[3]) D.corr[tr] = cf;
+    const double cf = pending_corr(S, tr, red, pending);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && pending) D.corr[tr] = cf;
     if (!active) continue;
     acc += cf * gw;
     Qt[tgt] = (T)acc;
@@ -234,7 +236,7 @@ __global__ void k_settle(Solve *solves, int first) {
   csadv_solve_desc &D = S.d;
   if (D.mf != 3 || !D.ctl[3]) return;
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const double cf = pending_corr(S, tr, red);
+    const double cf = pending_corr(S, tr, red, 1);
     if (threadIdx.x == 0) D.corr[tr] = cf;
   }
   __syncthreads();
@@ -261,20 +263,25 @@ struct StepIn {     // inputs of iteration y at the lane's column: row y and y-e
   T v, vs, gvp;              // y-edge y-3: wind, upwind-side wind; metric of y-edge y-2
 };
 
-template <typename T, int RECON, bool RK2>
+// SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07); the metric treatment follows from it, since
+// the valid pairs are (AVLT or L04, MT-0) and (PL07, MT-PL07) (src/discrete_operators.py:48-82).
+template <typename T, int RECON, bool RK2, int SPLIT>
 struct March {
+  static constexpr int split = SPLIT;
+  static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
   const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
   const double *gc64;
   T *Qo;
-  int ld, ldu, M, iend, x, xc, xe, tj, ylast, split;
-  bool out_lane, in_col, need_col, mt0, mfpr;
+  int ld, ldu, M, iend, x, xc, xe, tj, ylast;
+  bool out_lane, in_col, need_col, mfpr;
   T dtdx, dtdy;
   double cf;
   // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6)
   T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6];
   T gvm1, gv0, gvp1;         // gv at y-edges e-1, e, e+1 (e = y-3)
   T g1prev, g2prev, vprev;
+  T pa_A, pa_B, pa_q6, pb_A, pb_B, pb_q6;   // row y-4 as the left (upwind) cell of edge y-3: Q, Qx
   double part;
 
   __device__ __forceinline__ void load(int y, StepIn<T> &r) const {
@@ -309,54 +316,59 @@ struct March {
     gvm1 = gv0;
     gv0 = gvp1;
     gvp1 = in.gvp;
-    // ---- column stage: y-edge e = y-3, output row r = y-4
-    if (y >= tj + 3) {
-      const T v = in.v;
-      const bool vup = in.vs >= T(0);
-      const T gcm = gcw[s2], gce = gcw[s3];   // rows e-1 = y-4 and e = y-3
-      const T cy = v * dtdy;
-      const T g1 = edge_flux<T, RECON>(qw[s0], qw[s1], qw[s2], qw[s3], qw[s4], qw[s5], vup, cy, v, gvm1, gv0, gvp1,
-                                       gcm, gce, mt0);
-      const T g2 = edge_flux<T, RECON>(xw[s0], xw[s1], xw[s2], xw[s3], xw[s4], xw[s5], vup, cy, v, gvm1, gv0, gvp1,
-                                       gcm, gce, mt0);
-      if (y >= tj + 4) {
-        const T dG1 = (g1prev - g1) * dtdy;
-        T d1 = T(0), d2 = T(0);
-        if (split != 1) {
-          d1 = gv0 * cy;
-          d2 = gvm1 * (vprev * dtdy);
+    // ---- column stage: parabolas of row y-3 (from y = tj+2 on), y-edge e = y-3 (from tj+3 on),
+    // output row r = y-4 (from tj+4 on)
+    if (y >= tj + 2) {
+      const T gce = gcw[s3];
+      const Parab<T> a = parabola<T, RECON>(qw[s1], qw[s2], qw[s3], qw[s4], qw[s5], gv0, gvp1, gce, mt0);
+      const Parab<T> b = parabola<T, RECON>(xw[s1], xw[s2], xw[s3], xw[s4], xw[s5], gv0, gvp1, gce, mt0);
+      if (y >= tj + 3) {
+        const T v = in.v;
+        const bool vup = in.vs >= T(0);
+        const T cy = v * dtdy;
+        const T g1 = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, v, gv0, mt0);
+        const T g2 = upwind<T>(vup, pb_A, pb_B, pb_q6, b, cy, v, gv0, mt0);
+        if (y >= tj + 4) {
+          const T gcm = gcw[s2];               // row y-4
+          const T dG1 = (g1prev - g1) * dtdy;
+          T d1 = T(0), d2 = T(0);
+          if (split != 1) {
+            d1 = gv0 * cy;
+            d2 = gvm1 * (vprev * dtdy);
+          }
+          const T rg = rgw[s2];
+          const T qy = split_update<T>(split, qw[s2], dG1, gcm, rg, d1, d2);
+          const T ur = uw[s2];
+          const bool urp = usw[s2] >= T(0);
+          const T gur = guw[s2];
+          const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
+          const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
+          const Parab<T> c = parabola<T, RECON>(qy_m2, qy_m1, qy, qy_p1, qy_p2, gur, from_right(gur), gcm, mt0);
+          const T f2 = upwind<T>(urp, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, ur * dtdx, ur, gur, mt0);
+          const T dF2 = (f2 - from_right(f2)) * dtdx;
+          const T dG2 = (g2prev - g2) * dtdy;
+          // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
+          const T flx = dF2 + dG2;
+          if (out_lane && y <= ylast) {
+            st_at(Qo, (uint32_t)((y - 4) * ld + x), qw[s2] + flx * rg);
+            if (mfpr) part -= (double)flx;
+          }
         }
-        const T rg = rgw[s2];
-        const T qy = split_update<T>(split, qw[s2], dG1, gcm, rg, d1, d2);
-        const T ur = uw[s2];
-        const bool urp = usw[s2] >= T(0);
-        const T gur = guw[s2];
-        const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
-        const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
-        const T qy_m3 = from_left(qy_m2);
-        const T f2 = edge_flux<T, RECON>(qy_m3, qy_m2, qy_m1, qy, qy_p1, qy_p2, urp, ur * dtdx, ur, from_left(gur), gur,
-                                         from_right(gur), from_left(gcm), gcm, mt0);
-        const T dF2 = (f2 - from_right(f2)) * dtdx;
-        const T dG2 = (g2prev - g2) * dtdy;
-        // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
-        const T flx = dF2 + dG2;
-        if (out_lane && y <= ylast) {
-          st_at(Qo, (uint32_t)((y - 4) * ld + x), qw[s2] + flx * rg);
-          if (mfpr) part -= (double)flx;
-        }
+        g1prev = g1;
+        g2prev = g2;
+        vprev = v;
       }
-      g1prev = g1;
-      g2prev = g2;
-      vprev = v;
+      pa_A = a.rA; pa_B = a.rB; pa_q6 = a.q6;
+      pb_A = b.rA; pb_B = b.rB; pb_q6 = b.q6;
     }
     // ---- row stage: row y
     const T q = in.q, u = in.u, gu0 = in.gu, gc0 = in.gc;
     const bool up = in.us >= T(0);
-    const T gum1 = from_left(gu0), gup1 = from_right(gu0), gcm1 = from_left(gc0);
+    const T gup1 = from_right(gu0);
     const T q_m1 = from_left(q), q_p1 = from_right(q);
     const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
-    const T q_m3 = from_left(q_m2);
-    const T f1 = edge_flux<T, RECON>(q_m3, q_m2, q_m1, q, q_p1, q_p2, up, u * dtdx, u, gum1, gu0, gup1, gcm1, gc0, mt0);
+    const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
+    const T f1 = upwind<T>(up, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, u * dtdx, u, gu0, mt0);
     const T dF1 = (f1 - from_right(f1)) * dtdx;
     T c1 = T(0), c2 = T(0);
     if (split != 1) {
@@ -372,8 +384,9 @@ struct March {
   __device__ __forceinline__ void run() {
 #pragma unroll
     for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = T(0);
-    gvm1 = gv0 = gvp1 = T(0);     // complete from y = tj-1 on; the column stage starts at tj+3
+    gvm1 = gv0 = gvp1 = T(0);     // complete from y = tj-1 on; the column stage starts at tj+2
     g1prev = g2prev = vprev = T(0);
+    pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
     part = 0.0;
     StepIn<T> b0, b1;
     const int y0 = tj - 3;
@@ -392,7 +405,7 @@ struct March {
   }
 };
 
-template <typename T, int RECON, bool RK2>
+template <typename T, int RECON, bool RK2, int SPLIT>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *work) {
   const int lane = threadIdx.x;
   const int4 wk = work[blockIdx.x];
@@ -412,7 +425,7 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
     D.ctl[1] = step + 1;
     if (mfpr) D.ctl[3] = 1;
   }
-  March<T, RECON, RK2> m;
+  March<T, RECON, RK2, SPLIT> m;
   m.ld = D.ld;
   m.ldu = D.ldu;
   m.M = S.M;
@@ -422,11 +435,9 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   m.xe = min(m.x, m.M);
   m.tj = tj;
   m.ylast = tj + ny + 3;
-  m.split = D.split;
   m.out_lane = lane >= 3 && lane < 3 + nx;
   m.in_col = m.x >= NG && m.x < iend;
   m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
-  m.mt0 = D.mt == 1;
   m.mfpr = mfpr;
   m.dtdx = T(D.dt / D.dx);
   m.dtdy = T(D.dt / D.dy);
@@ -724,6 +735,7 @@ struct csadv_s {
   int dtype = 0;
   int recon = 3;
   bool rk2 = false;               // dp = 2 (separate upwind-side winds)
+  int split = 1;                  // splitting of every solve (its metric treatment follows)
   bool general = false;           // ET-S72 / ET-PL07 / MF-AF: the general step
   double *red = nullptr;          // small reduction scratch (mass / norms)
   std::vector<int> done;          // host copy of the steps completed per solve (ctl[1])
@@ -822,13 +834,20 @@ Solve derive(const csadv_solve_desc &d, int tile) {
   return s;
 }
 
+template <typename T, int R, int SP>
+hipError_t launch_step_s(csadv_t *h, hipStream_t st) {
+  if (h->rk2)
+    hipLaunchKernelGGL((k_step<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  else
+    hipLaunchKernelGGL((k_step<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  return hipGetLastError();
+}
+
 template <typename T, int R>
 hipError_t launch_step_r(csadv_t *h, hipStream_t st) {
-  if (h->rk2)
-    hipLaunchKernelGGL((k_step<T, R, true>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
-  else
-    hipLaunchKernelGGL((k_step<T, R, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
-  return hipGetLastError();
+  if (h->split == 2) return launch_step_s<T, R, 2>(h, st);
+  if (h->split == 3) return launch_step_s<T, R, 3>(h, st);
+  return launch_step_s<T, R, 1>(h, st);
 }
 
 template <typename T>
@@ -928,15 +947,16 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
   h->dtype = solves[0].dtype;
   h->recon = solves[0].recon;
   h->rk2 = solves[0].dp == 2;
+  h->split = solves[0].split;
   h->general = solves[0].et != 3 || solves[0].mf == 2;
   for (int s = 0; s < nsolve; ++s) {
     std::string msg;
     int rc = validate(solves[s], msg);
-    if (rc == CSADV_OK && (solves[s].dtype != h->dtype || solves[s].recon != h->recon ||
+    if (rc == CSADV_OK && (solves[s].dtype != h->dtype || solves[s].recon != h->recon || solves[s].split != h->split ||
                            solves[s].dp != solves[0].dp ||
                            (solves[s].et != 3 || solves[s].mf == 2) != h->general)) {
       rc = CSADV_E_ARG;
-      msg = "all solves of one handle share dtype, recon, dp and the step kind (fused or general)";
+      msg = "all solves of one handle share dtype, recon, split, dp and the step kind (fused or general)";
     }
     if (rc != CSADV_OK) {
       // the handle is returned so that csadv_last_error() can explain the failure

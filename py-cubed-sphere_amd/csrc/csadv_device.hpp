@@ -108,49 +108,38 @@ __device__ __forceinline__ void recon_edges(T m2, T m1, T c0, T p1, T p2, T &qL,
   }
 }
 
-// Upwind PPM flux through one edge (src/flux.py:20-70).  w points at the upwind cell in a
-// row/column with element stride s; upos selects the left cell (flux from the left).
-// cc = Courant number, uav = time-averaged wind, gl / gr / gcell = metric at the cell's
-// left edge, right edge and centre, ge = metric at the flux edge (MT-PL07).
+// The parabola of one cell as the upwind flux uses it (src/flux.py:28-44): edge values
+// from the 5-cell window, with the MT-0 metric applied (gl / gr at the cell's left and right
+// edges, gcell at its centre), dq = q_R - q_L and q6 = 6q - 3(q_R + q_L).  An edge takes it
+// from the cell on its left when the wind there is non-negative (rA = q_R, rB = q6 - dq)
+// and from the cell on its right otherwise (lA = q_L, lB = q6 + dq).  Each cell's parabola
+// is formed once and an edge only selects three values from one side.
+template <typename T>
+struct Parab {
+  T rA, rB, lA, lB, q6;
+};
+
 template <typename T, int RECON>
-__device__ __forceinline__ T upwind_flux(const T *w, int s, bool upos, T cc, T uav, T gl, T gr, T gcell,
-                                         bool mt0, T ge) {
+__device__ __forceinline__ Parab<T> parabola(T m2, T m1, T c0, T p1, T p2, T gl, T gr, T gcell, bool mt0) {
   T qL, qR;
-  recon_edges<T, RECON>(w[-2 * s], w[-s], w[0], w[s], w[2 * s], qL, qR);
-  T q = w[0];
+  recon_edges<T, RECON>(m2, m1, c0, p1, p2, qL, qR);
+  T q = c0;
   if (mt0) { qL = qL * gl; qR = qR * gr; q = q * gcell; }
   const T dq = qR - qL;
   const T q6 = T(6) * q - T(3) * (qR + qL);
-  const T third = T(1.0 / 3.0);
-  T f = upos ? (qR + (cc * T(0.5)) * (q6 - dq) - ((q6 * cc) * cc) * third)
-             : (qL - (cc * T(0.5)) * (q6 + dq) - ((q6 * cc) * cc) * third);
-  f = f * uav;
-  if (!mt0) f = f * ge;
-  return f;
+  return {qR, q6 - dq, qL, q6 + dq, q6};
 }
 
-// Same flux from the 6-cell neighbourhood of the edge between cells c-1 and c:
-// m3..p2 = Q[c-3..c+2]; the upwind cell (c-1 if up, else c) supplies the parabola.
-// ge_m1 / ge_0 / ge_p1: metric at edges c-1, c, c+1; gc_m1 / gc_0: metric at cells c-1, c.
-template <typename T, int RECON>
-__device__ __forceinline__ T edge_flux(T m3, T m2, T m1, T c0, T p1, T p2, bool up, T cc, T uav,
-                                       T ge_m1, T ge_0, T ge_p1, T gc_m1, T gc_0, bool mt0) {
-  const T w0 = up ? m3 : m2, w1 = up ? m2 : m1, w2 = up ? m1 : c0, w3 = up ? c0 : p1, w4 = up ? p1 : p2;
-  T qL, qR;
-  recon_edges<T, RECON>(w0, w1, w2, w3, w4, qL, qR);
-  T q = w2;
-  if (mt0) {
-    qL = qL * (up ? ge_m1 : ge_0);
-    qR = qR * (up ? ge_0 : ge_p1);
-    q = q * (up ? gc_m1 : gc_0);
-  }
-  const T dq = qR - qL;
-  const T q6 = T(6) * q - T(3) * (qR + qL);
-  const T third = T(1.0 / 3.0);
-  T f = up ? (qR + (cc * T(0.5)) * (q6 - dq) - ((q6 * cc) * cc) * third)
-           : (qL - (cc * T(0.5)) * (q6 + dq) - ((q6 * cc) * cc) * third);
+// Upwind PPM flux through one edge (src/flux.py:48-70) from the parabolas of the cells on its
+// left (L) and right (R): f = q_R + (c/2)(q6 - dq) - q6 c^2/3 from the left cell when up,
+// f = q_L - (c/2)(q6 + dq) - q6 c^2/3 from the right cell otherwise; F = f u_avg, and
+// MT-PL07 multiplies by the edge metric ge.
+template <typename T>
+__device__ __forceinline__ T upwind(bool up, T LrA, T LrB, T Lq6, const Parab<T> &R, T cc, T uav, T ge, bool mt0) {
+  const T A = up ? LrA : R.lA, B = up ? LrB : -R.lB, q6 = up ? Lq6 : R.q6;
+  T f = A + (cc * T(0.5)) * B - ((q6 * cc) * cc) * T(1.0 / 3.0);
   f = f * uav;
-  if (!mt0) f = f * ge_0;
+  if (!mt0) f = f * ge;
   return f;
 }
 

@@ -21,7 +21,7 @@ from .solver import Advection, cfl_dt, grid_tables, interior_winds
 class Batch:
     """One handle over ``members`` (constructed :class:`Advection` solves, not initialised).
 
-    All members share dtype, recon and dp (one kernel instantiation per launch)."""
+    All members share dtype, recon, split and dp (one kernel instantiation per launch)."""
 
     def __init__(self, members, graph_steps=16):
         if not members:
