@@ -22,7 +22,8 @@ time-dependent winds).
 Prints ONE JSON line:
   value        tracer cell-updates/s over the timed steps (inputs resident in HBM)
   roofline     the step kernel against the HBM roofline (algorithmic bytes per launch
-               over its event-timed mean duration); traffic from the committed rocprofv3
+               over its mean duration: HIP events on the launch stream around one graph
+               of 200 back-to-back launches); traffic from the committed rocprofv3
                counter summary of the workload when present
   cpu_baseline the numpy FP64 oracle on this host, one core, on a bounded sample of the
                same workload (a few steps of one solve), scaled to cell-updates/s
@@ -218,7 +219,9 @@ def main():
     bytes_per_launch = sum(algorithmic_bytes_per_update(dtype, m.case.ntr) * 6.0 * m.grid.N ** 2 * m.case.ntr
                            for m in members)
     b_alg = bytes_per_launch / per_step
-    kern_s = kern_ms / 1e3 / max(launches, 1)
+    in_step_s = kern_ms / 1e3 / max(launches, 1)       # event pair around each launch inside the step
+    burst_reps = 200
+    kern_s = stepper.time_step_kernel(burst_reps) * 1e-6   # one event pair around a graph of launches
     achieved = bytes_per_launch / kern_s / 1e9
     Ns = sorted({m.grid.N for m in members})
 
@@ -250,7 +253,10 @@ def main():
             "kernel": "k_step",
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "mean_launch_us": kern_s * 1e6,
-            "launches_timed": launches,
+            "launches_timed": burst_reps,
+            "timing": "HIP events on the launch stream around one graph of %d back-to-back step-kernel "
+                      "launches (each redoes the same step)" % burst_reps,
+            "mean_launch_us_in_step": in_step_s * 1e6,
         },
     }
     if tuned is not None:

@@ -167,6 +167,13 @@ int csadv_current(csadv_t *h, int32_t solve);
 int csadv_profile(csadv_t *h, int32_t capacity);
 int csadv_profile_read(csadv_t *h, double *total_ms, int32_t *launches);
 
+/* Mean duration in microseconds of the step kernel alone, over reps (1..4096) back-to-back
+ * launches replayed from one HIP graph between one event pair on `stream`.  The repeats
+ * redo the same step (same input, same output), so the state afterwards equals that of one
+ * launch.  Fused step only (CSADV_E_STATE for a general-step handle).  No counterpart in the
+ * reference: measurement of the kernel's roofline. */
+int csadv_time_step_kernel(csadv_t *h, int32_t reps, void *stream, double *us_per_launch);
+
 const char *csadv_last_error(csadv_t *h);
 void csadv_destroy(csadv_t *h);
 int csadv_abi_version(void);

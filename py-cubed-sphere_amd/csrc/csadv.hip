@@ -1366,6 +1366,44 @@ int csadv_profile_read(csadv_t *h, double *total_ms, int32_t *launches) {
   return CSADV_OK;
 }
 
+// Mean duration of the step kernel over reps back-to-back launches captured in one graph
+// and bracketed by one event pair (measurement only).  The step kernel takes its step index
+// from ctl[2], which only the ghost fill advances, so every repeat reads the same input and
+// writes the same output: the state afterwards is that of a single launch.
+int csadv_time_step_kernel(csadv_t *h, int32_t reps, void *stream, double *us_per_launch) {
+  if (!h || !us_per_launch || reps < 1 || reps > 4096) return CSADV_E_ARG;
+  if (h->general) return fail(h, CSADV_E_STATE, "the general step has no single step kernel to time");
+  hipStream_t st = (hipStream_t)stream;
+  if (!h->cap) HIPCHK(h, hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking));
+  HIPCHK(h, hipStreamBeginCapture(h->cap, hipStreamCaptureModeRelaxed));
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < reps && e == hipSuccess; ++k)
+    e = h->dtype == CSADV_F64 ? launch_step_dt<double>(h, h->cap) : launch_step_dt<float>(h, h->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->cap, &g);
+  if (e != hipSuccess || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(h, CSADV_E_HIP, std::string("timing capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(h, CSADV_E_HIP, std::string("timing graph: ") + hipGetErrorString(e));
+  hipEvent_t a = nullptr, b = nullptr;
+  float ms = 0.f;
+  if ((e = hipEventCreate(&a)) == hipSuccess && (e = hipEventCreate(&b)) == hipSuccess &&
+      (e = hipGraphLaunch(x, st)) == hipSuccess &&                          // warm-up replay
+      (e = hipEventRecord(a, st)) == hipSuccess && (e = hipGraphLaunch(x, st)) == hipSuccess &&
+      (e = hipEventRecord(b, st)) == hipSuccess && (e = hipEventSynchronize(b)) == hipSuccess)
+    e = hipEventElapsedTime(&ms, a, b);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  (void)hipGraphExecDestroy(x);
+  if (e != hipSuccess) return fail(h, CSADV_E_HIP, std::string("timing run: ") + hipGetErrorString(e));
+  *us_per_launch = 1e3 * (double)ms / reps;
+  return CSADV_OK;
+}
+
 const char *csadv_last_error(csadv_t *h) { return h ? h->err.c_str() : "null handle"; }
 
 void csadv_destroy(csadv_t *h) {

@@ -106,6 +106,14 @@ class Batch:
                    "csadv_profile_read")
         return ms.value, n.value
 
+    def time_step_kernel(self, reps=200):
+        """Mean microseconds of the step kernel alone over ``reps`` graph-replayed launches
+        (the repeats redo the current step, so the state is unchanged by the call)."""
+        us = ctypes.c_double()
+        _lib.check(self.lib.csadv_time_step_kernel(self.handle, int(reps), self._stream(), ctypes.byref(us)), self.handle,
+                   "csadv_time_step_kernel")
+        return us.value
+
     def use_graph(self, steps_per_graph):
         _lib.check(self.lib.csadv_set_graph(self.handle, int(steps_per_graph)), self.handle, "csadv_set_graph")
 

@@ -359,6 +359,14 @@ class Advection:
                    "csadv_profile_read")
         return ms.value, n.value
 
+    def time_step_kernel(self, reps=200):
+        """Mean microseconds of the step kernel alone over ``reps`` graph-replayed launches
+        (the repeats redo the current step, so the state is unchanged by the call)."""
+        us = ctypes.c_double()
+        _lib.check(self.lib.csadv_time_step_kernel(self.handle, int(reps), self._stream(), ctypes.byref(us)), self.handle,
+                   "csadv_time_step_kernel")
+        return us.value
+
     def close(self):
         if self.handle and self.owner is None:
             self.lib.csadv_destroy(self.handle)
