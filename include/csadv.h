@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CSADV_ABI_VERSION 3
+#define CSADV_ABI_VERSION 4
 
 enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
 
@@ -83,7 +83,7 @@ typedef struct csadv_solve_desc {
   const double *corner_w;          /* [n_corner][16]                                        */
   const double *corner_gw;         /* [n_corner]                                            */
   const double *coef;              /* [max_steps][2][nbasis] basis coefficients (t_{n-1}, t_{n-2}) */
-  double *corr;                    /* [ntr] pending mass-fixer correction (FP64)        */
+  double *corr;                    /* [2][ntr] MF-PR correction totals C (FP64), slot by step parity: Q = S + C g */
   double *partials;                /* scratch, >= ntr * csadv_partials_per_tracer()     */
   int32_t *ctl;                    /* [4] device control words: cur, step, 0, 0          */
   int32_t tile;                    /* step tile shape of this solve (csadv_tile_rows);
@@ -143,9 +143,11 @@ int csadv_step(csadv_t *h, int32_t nsteps, void *stream);
  * and releases the graph.  No counterpart in the reference (its step is a Python call). */
 int csadv_set_graph(csadv_t *h, int32_t steps_per_graph);
 
-/* Fold the pending mass-fixer corrections of one solve (solve = -1: every solve) into its
+/* Fold the mass-fixer correction totals of one solve (solve = -1: every solve) into its
  * current state buffer so that it holds Q exactly as the reference would after the last
- * step.  Other solves of the handle are left untouched, so reading one result never
+ * step.  During stepping the stored interior values S stay uncorrected (Q = S + C g with an
+ * FP64 total C per tracer), so that an FP32 state does not round the small per-step
+ * corrections away.  Other solves of the handle are left untouched, so reading one result never
  * changes the arithmetic of the solves still running. */
 int csadv_finalize(csadv_t *h, int32_t solve, void *stream);
 

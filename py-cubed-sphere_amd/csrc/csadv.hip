@@ -55,7 +55,7 @@ __global__ void k_ghost_edges(const FieldSet *sets) {
     const int64_t tgt = (int64_t)p * F.pstride + (int64_t)tj * ld + ti;
     for (int f = 0; f < F.nfield; ++f) {
       T *Q = base + (int64_t)f * F.fstride;
-      const double cf = F.corr ? F.corr[f] : 0.0;
+      const double cf = F.corr ? F.corr[(F.cur ? (*F.cur & 1) * F.nfield : 0) + f] : 0.0;
       double acc = 0.0;
 #pragma unroll
       for (int l = 0; l < 4; ++l) acc += ((double)Q[src[l]] + cf * gs[l]) * w[l];
@@ -95,7 +95,7 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
   const int64_t tgt = (int64_t)p * F.pstride + (int64_t)tj * ld + ti;
   for (int f = 0; f < F.nfield; ++f) {
     T *Q = base + (int64_t)f * F.fstride;
-    const double cf = F.corr ? F.corr[f] : 0.0;
+    const double cf = F.corr ? F.corr[(F.cur ? (*F.cur & 1) * F.nfield : 0) + f] : 0.0;
     double acc = 0.0;
 #pragma unroll
     for (int l = 0; l < 4; ++l) acc += ((double)Q[src[l]] + cf * gs[l]) * w[l];
@@ -113,8 +113,12 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
 // only and read by the other, so no kernel reads a word it writes:
 //   ctl[1]  steps completed           written by k_step (one lane), read by k_ghost, k_wind_avg
 //   ctl[2]  index of the running step written by k_ghost (one lane), read by k_step
-//   ctl[3]  1 = the partials of the last step are not yet folded into corr
-// The state of step n lives in q[n & 1].
+//   ctl[3]  1 = the partials of the last step are not yet added to the correction total
+// The state of step n lives in q[n & 1].  With MF-PR the stored interior values S are not
+// corrected during stepping: Q = S + C g, where C (FP64, per tracer) is the running total of
+// the mass-fixer corrections, kept in corr[(n & 1) * ntr + tracer] for step n.  Folding a
+// correction of ~1e-9 into an FP32 state at every step would round it away where |S| is
+// large and keep it where S is small, a bias that grows with the step count.
 __device__ __forceinline__ double block_sum_fixed(double v, double *red) {
   red[threadIdx.x] = v;
   __syncthreads();
@@ -127,16 +131,18 @@ __device__ __forceinline__ double block_sum_fixed(double v, double *red) {
   return s;
 }
 
-// Mass-fixer correction of the last step: c = dt * m0 / a2 with m0 = sum of div g over the
-// interior (src/discrete_operators.py:98-101), summed in a fixed order so that every block
-// (and every run) gets the same bits.  pending = ctl[3], read by the caller up front: the
-// partial sums are requested whether or not they are pending, so that no load waits on the
-// flag; every thread issues its eight loads of a batch before adding them in the order
-// i, i+BLOCK, ... (masked past the end, so no thread walks its tail one load at a time).
-__device__ double pending_corr(const Solve &S, int tr, double *red, int pending) {
+// Mass-fixer correction total C after `steps` steps.  With the partials of the last step
+// pending (pending = ctl[3], read by the caller up front), C = C(steps - 1) + dt m0 / a2,
+// m0 = sum of div g over the interior (src/discrete_operators.py:98-101) summed in a fixed
+// order so that every block (and every run) gets the same bits; otherwise C is the stored
+// total of this step.  The partial sums are requested whether or not they are pending, so
+// that no load waits on the flag; every thread issues its eight loads of a batch before
+// adding them in the order i, i+BLOCK, ... (masked past the end).
+__device__ double pending_corr(const Solve &S, int tr, double *red, int pending, int steps) {
   const csadv_solve_desc &D = S.d;
   if (D.mf != 3) return 0.0;
-  const double held = D.corr[tr];
+  const double held = D.corr[(steps & 1) * D.ntr + tr];
+  const double prev = D.corr[((steps + 1) & 1) * D.ntr + tr];
   const double *P = D.partials + (int64_t)tr * S.part_per_tracer;
   const int n = S.part_per_tracer;
   double acc = 0.0;
@@ -149,7 +155,7 @@ __device__ double pending_corr(const Solve &S, int tr, double *red, int pending)
       if (i + u * BLOCK < n) acc += v[u];
   }
   const double sum = block_sum_fixed(acc, red);
-  return pending ? D.dt * sum / D.a2 : held;
+  return pending ? prev + D.dt * sum / D.a2 : held;
 }
 
 template <typename T>
@@ -220,8 +226,8 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
         for (int l = 0; l < 16; ++l) acc += (double)v[l] * w16[l];
       }
     }
-    const double cf = pending_corr(S, tr, red, pending);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && pending) D.corr[tr] = cf;
+    const double cf = pending_corr(S, tr, red, pending, steps);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && pending) D.corr[(steps & 1) * D.ntr + tr] = cf;
     if (!active) continue;
     acc += cf * gw;
     Qt[tgt] = (T)acc;
@@ -229,15 +235,17 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
   if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
 }
 
-// Fold the last step's partials into corr (once; used by finalize and the diagnostics).
+// Add the last step's partials to the correction total (once; used by finalize and the
+// diagnostics).
 __global__ void k_settle(Solve *solves, int first) {
   __shared__ double red[BLOCK];
   Solve &S = solves[first + blockIdx.x];
   csadv_solve_desc &D = S.d;
   if (D.mf != 3 || !D.ctl[3]) return;
+  const int steps = D.ctl[1];
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const double cf = pending_corr(S, tr, red, 1);
-    if (threadIdx.x == 0) D.corr[tr] = cf;
+    const double cf = pending_corr(S, tr, red, 1, steps);
+    if (threadIdx.x == 0) D.corr[(steps & 1) * D.ntr + tr] = cf;
   }
   __syncthreads();
   if (threadIdx.x == 0) D.ctl[3] = 0;
@@ -260,6 +268,7 @@ __global__ void k_settle(Solve *solves, int first) {
 template <typename T>
 struct StepIn {     // inputs of iteration y at the lane's column: row y and y-edge y-3
   T q, u, us, gu, gc, rg;    // row y: Q, x-edge wind, upwind-side wind, metric, 1/g
+  T raw;                     // row y: the stored value before the correction total is added
   T v, vs, gvp;              // y-edge y-3: wind, upwind-side wind; metric of y-edge y-2
 };
 
@@ -279,6 +288,7 @@ struct March {
   double cf;
   // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6)
   T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6];
+  T sw[6];                   // FP32: stored values S of the rows (the update is S + flux)
   T gvm1, gv0, gvp1;         // gv at y-edges e-1, e, e+1 (e = y-3)
   T g1prev, g2prev, vprev;
   T pa_A, pa_B, pa_q6, pb_A, pb_B, pb_q6;   // row y-4 as the left (upwind) cell of edge y-3: Q, Qx
@@ -300,8 +310,9 @@ struct March {
     r.gvp = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
   }
 
-  // interior cells carry the pending mass-fixer correction (ghosts got theirs in k_ghost)
+  // interior cells carry the mass-fixer correction total (ghosts got theirs in k_ghost)
   __device__ __forceinline__ void correct(int y, StepIn<T> &r) const {
+    r.raw = r.q;
     if (cf != 0.0 && in_col && y >= NG && y < iend) {
       const double g = std::is_same<T, double>::value ? (double)r.gc : ld_at(gc64, (uint32_t)(y * ld + x));
       r.q = (T)((double)r.q + cf * g);
@@ -350,7 +361,11 @@ struct March {
           // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
           const T flx = dF2 + dG2;
           if (out_lane && y <= ylast) {
-            st_at(Qo, (uint32_t)((y - 4) * ld + x), qw[s2] + flx * rg);
+            // the correction total stays out of the stored state: S(n+1) = S(n) + flux
+            T out;
+            if constexpr (std::is_same<T, float>::value) out = sw[s2] + flx * rg;
+            else out = (qw[s2] + flx * rg) - T(cf) * gcm;
+            st_at(Qo, (uint32_t)((y - 4) * ld + x), out);
             if (mfpr) part -= (double)flx;
           }
         }
@@ -378,12 +393,13 @@ struct March {
     const T qx = split_update<T>(split, q, dF1, gc0, in.rg, c1, c2);
     // row y replaces row y-6 in every ring
     qw[s0] = q; xw[s0] = qx; gcw[s0] = gc0; rgw[s0] = in.rg; uw[s0] = u; usw[s0] = in.us; guw[s0] = gu0;
+    if constexpr (std::is_same<T, float>::value) sw[s0] = in.raw;
     if (y + 2 <= ylast) load(y + 2, in);
   }
 
   __device__ __forceinline__ void run() {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = T(0);
+    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = sw[k] = T(0);
     gvm1 = gv0 = gvp1 = T(0);     // complete from y = tj-1 on; the column stage starts at tj+2
     g1prev = g2prev = vprev = T(0);
     pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
@@ -455,7 +471,7 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
   m.Q = Qin + tr * S.tstride + p * S.pstride;
   m.Qo = Qout + tr * S.tstride + p * S.pstride;
-  m.cf = D.corr ? D.corr[tr] : 0.0;
+  m.cf = D.corr ? D.corr[(step & 1) * D.ntr + tr] : 0.0;
   m.run();
   if (mfpr) {
     const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
@@ -463,7 +479,8 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   }
 }
 
-// Fold pending corrections into the current buffer (interior only) and clear them.
+// Fold the correction totals into the current buffer (interior only); k_clear_corr then
+// clears them.
 template <typename T>
 __global__ void k_finalize(Solve *solves, int first) {
   Solve &S = solves[first + blockIdx.y];
@@ -473,7 +490,7 @@ __global__ void k_finalize(Solve *solves, int first) {
   T *Q = reinterpret_cast<T *>((D.ctl[1] & 1) ? D.q1 : D.q0);
   const int64_t tot = (int64_t)6 * N * N;
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const double cf = D.corr[tr];
+    const double cf = D.corr[(D.ctl[1] & 1) * D.ntr + tr];
     if (cf == 0.0) continue;
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
       const int p = (int)(idx / ((int64_t)N * N));
@@ -488,7 +505,7 @@ __global__ void k_finalize(Solve *solves, int first) {
 __global__ void k_clear_corr(Solve *solves, int first) {
   csadv_solve_desc &D = solves[first + blockIdx.x].d;
   if (D.mf == 3)
-    for (int tr = threadIdx.x; tr < D.ntr; tr += blockDim.x) D.corr[tr] = 0.0;
+    for (int k = threadIdx.x; k < 2 * D.ntr; k += blockDim.x) D.corr[k] = 0.0;
 }
 
 // ------------------------------------------------------------------ winds
@@ -663,7 +680,7 @@ __global__ void k_mass(const Solve *solves, int solve, int tracer, double *parti
   const csadv_solve_desc &D = S.d;
   const int N = D.N, ld = D.ld;
   const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
-  const double cf = D.corr ? D.corr[tracer] : 0.0;
+  const double cf = D.corr ? D.corr[(D.ctl[1] & 1) * D.ntr + tracer] : 0.0;
   const double dxdy = D.dx * D.dy;
   KSum acc{0.0, 0.0};
   const int64_t tot = (int64_t)6 * N * N;
@@ -693,7 +710,7 @@ __global__ void k_norms(const Solve *solves, int solve, int tracer, const double
   const csadv_solve_desc &D = S.d;
   const int N = D.N, ld = D.ld;
   const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
-  const double cf = D.corr ? D.corr[tracer] : 0.0;
+  const double cf = D.corr ? D.corr[(D.ctl[1] & 1) * D.ntr + tracer] : 0.0;
   KSum s1{0.0, 0.0}, s2{0.0, 0.0};
   double mx = 0.0;
   const int64_t tot = (int64_t)6 * N * N;
@@ -1096,7 +1113,9 @@ int csadv_wind_init(csadv_t *h, void *stream) {
 
 int csadv_ghost_fill(csadv_t *h, void *stream) {
   if (!h || !h->dev) return CSADV_E_STATE;
-  return ghost_fill(h, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_settle, dim3((int)h->desc.size()), dim3(BLOCK), 0, st, h->dev, 0);   // totals current
+  return ghost_fill(h, st);
 }
 
 }  // extern "C"

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcsadv.so")
 
 CSADV_F64, CSADV_F32 = 0, 1
-ABI_VERSION = 3          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
+ABI_VERSION = 4          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
 _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of order", -4: "size out of range"}
 
 # exported symbols, in declaration order of include/csadv.h

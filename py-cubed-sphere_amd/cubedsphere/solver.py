@@ -193,7 +193,7 @@ class Advection:
         self.wpc = torch.zeros((K, 2, 6, M, ld), **f64)
         steps = 1 if self.case.steady else self.max_steps
         self.coef = torch.from_numpy(self.case.coefficient_table(steps, self.dt).reshape(-1)).to(**f64)
-        self.corr = torch.zeros(ntr, **f64)
+        self.corr = torch.zeros(2 * ntr, **f64)      # MF-PR correction totals, one slot per step parity
         nparts = self.lib.csadv_partials_per_tracer(N, self.tile)
         _lib.check(nparts, None, "csadv_partials_per_tracer")
         self.partials = torch.zeros(ntr * nparts, **f64)

@@ -120,3 +120,29 @@ def test_device_built_preparation_matches_host(N):
     qa, qb = a.state(), b.state()
     for n in range(case.ntr):
         assert np.abs(qa[n] - qb[n]).max() <= 1e-12 * np.abs(qa[n]).max(), n
+
+
+def test_fp32_mass_fixer_keeps_mass_and_tracks_fp64():
+    """MF-PR's correction is a running FP64 total kept outside the FP32 state (Q = S + C g), so an
+    FP32 solve with the fixer stays as close to FP64 as one without it, and keeps its mass.
+    Folding the ~1e-9 per-step corrections into the FP32 state instead rounds them away where
+    |q| is large: over 300 steps that drifted the mass by ~1e-6 and raised the L2 tenfold."""
+    from cubedsphere.batch import make_member
+    from cubedsphere.cases import Case
+    N, K = 96, 300
+    case = Case([{"field": "gaussians", "centres": [(0.7, 0.3)], "b": 20.0}], ("solid_body", -0.6, 5.0), "narrow")
+    l2, drift = {}, None
+    for mf in (3, 1):
+        out = {}
+        for dt_ in ("f32", "f64"):
+            m = make_member(N, case, K, dtype=dt_, mf=mf)
+            m.init_vars_adv()
+            m0 = m.mass()
+            m.step(K)
+            out[dt_] = m.state()[0]
+            if mf == 3 and dt_ == "f32":
+                drift = abs(m.mass() - m0) / abs(m0)
+            m.close()
+        l2[mf] = np.linalg.norm(out["f32"] - out["f64"]) / np.linalg.norm(out["f64"])
+    assert l2[3] <= 1.5e-6 and l2[3] <= 1.5 * l2[1] + 1e-7, l2
+    assert drift <= 2e-7, drift
