@@ -159,6 +159,17 @@ int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *str
  * host out = {Linf, L1, L2} as src/errors.py:99-113 (unweighted means).  Synchronises. */
 int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact, double *out, void *stream);
 
+/* Nearest-cell remap of one tracer onto the plotting lat-lon grid (src/interpolation.py:38-149,
+ * src/cs_datastruct.py:569-582): point (a, b) is lon = linspace(-pi, pi, nlon)[a],
+ * lat = linspace(-pi/2, pi/2, nlat)[b]; its panel is the face of its largest Cartesian component
+ * (a later face wins a tie, as the reference's mask order gives), its cell the floor of the
+ * inverse equiangular coordinates.  field == NULL remaps the tracer's state (with the mass-fixer
+ * total); otherwise field is a device FP64 [6][N][N] interior array (i fastest).  Device outputs,
+ * each [nlon][nlat] and each optional (NULL to skip): out (FP64 values, the reference's Q_ll),
+ * ix, jy, panel (int32, the reference's ll2cs tables).  nlat, nlon >= 2, nlat * nlon <= 2^31. */
+int csadv_remap_latlon(csadv_t *h, int32_t solve, int32_t tracer, const double *field, int32_t nlat, int32_t nlon,
+                       double *out, int32_t *ix, int32_t *jy, int32_t *panel, void *stream);
+
 /* Index (0 or 1) of the buffer holding the current state of a solve (host-tracked). */
 int csadv_current(csadv_t *h, int32_t solve);
 

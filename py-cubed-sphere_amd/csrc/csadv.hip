@@ -508,6 +508,64 @@ __global__ void k_clear_corr(Solve *solves, int first) {
     for (int k = threadIdx.x; k < 2 * D.ntr; k += blockDim.x) D.corr[k] = 0.0;
 }
 
+// ------------------------------------------------------------------ lat-lon output remap
+// Nearest-cell remap onto the plotting grid (src/cs_datastruct.py:569-582,
+// src/interpolation.py:38-149).  Point (a, b) has lon = -pi + a * (2pi / (nlon-1)) and
+// lat = -pi/2 + b * (pi / (nlat-1)), formed as numpy.linspace forms them (product, then sum,
+// each rounded; the last point is the end value).  Its panel is the face of its largest
+// Cartesian component; the reference assigns the face masks in the order 0..5, so a later face
+// wins a tie.  Its cell is floor((x + a)/dx), floor((y + a)/dy) of the inverse equiangular map
+// (clamped to the panel, where the reference would index out of range).  out[a][b] is the
+// tracer's value Q = S + C g (field == null) or field[p][j][i]; ix / jy / panel, when given,
+// receive the cell indices (the reference's ll2cs tables).
+template <typename T>
+__global__ void k_remap_latlon(const Solve *solves, int solve, int tracer, const double *field, int nlat, int nlon,
+                               double *out, int32_t *ix, int32_t *jy, int32_t *pn) {
+  const Solve &S = solves[solve];
+  const csadv_solve_desc &D = S.d;
+  const int N = D.N, ld = D.ld;
+  const double pi = 3.141592653589793, a = pi * 0.5 * 0.5, dx = 2.0 * a / N;
+  const double lon_step = (pi - (-pi)) / (nlon - 1), lat_step = (pi * 0.5 - (-pi * 0.5)) / (nlat - 1);
+  const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
+  const double cf = (D.corr && D.mf == 3) ? D.corr[(D.ctl[1] & 1) * D.ntr + tracer] : 0.0;
+  const int64_t total = (int64_t)nlon * nlat;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+    const int ia = (int)(k / nlat), ib = (int)(k - (int64_t)ia * nlat);
+    const double lon = ia == nlon - 1 ? pi : __dadd_rn(__dmul_rn((double)ia, lon_step), -pi);
+    const double lat = ib == nlat - 1 ? pi * 0.5 : __dadd_rn(__dmul_rn((double)ib, lat_step), -pi * 0.5);
+    const double cl = cos(lat);
+    const double X = __dmul_rn(cl, cos(lon)), Y = __dmul_rn(cl, sin(lon)), Z = sin(lat);
+    const double ax = fabs(X), ay = fabs(Y), az = fabs(Z), pm = fmax(fmax(ax, ay), az);
+    int p = 0;
+    if (pm == ax && X > 0) p = 0;
+    if (pm == ay && Y > 0) p = 1;
+    if (pm == ax && X < 0) p = 2;
+    if (pm == ay && Y < 0) p = 3;
+    if (pm == az && Z > 0) p = 4;
+    if (pm == az && Z <= 0) p = 5;
+    // inverse equiangular map of face p (src/cs_transform.py:99-126): tan x = nx / d, tan y = ny / d
+    const double d = (p == 0 || p == 2) ? X : ((p == 1 || p == 3) ? Y : Z);
+    const double nx = (p == 0 || p == 2) ? Y : ((p == 1 || p == 3) ? -X : (p == 4 ? Y : -Y));
+    const double ny = p == 0 ? Z : (p == 1 ? Z : (p == 2 ? -Z : (p == 3 ? -Z : -X)));
+    const double x = atan(nx / d), y = atan(ny / d);
+    const int i = min(max((int)floor((x - (-a)) / dx), 0), N - 1);
+    const int j = min(max((int)floor((y - (-a)) / dx), 0), N - 1);
+    if (out) {
+      double v;
+      if (field) {
+        v = field[((int64_t)p * N + j) * N + i];
+      } else {
+        const int ci = i + NG, cj = j + NG;
+        v = (double)Q[p * S.pstride + (int64_t)cj * ld + ci] + cf * D.gc64[(int64_t)cj * ld + ci];
+      }
+      out[k] = v;
+    }
+    if (ix) ix[k] = i;
+    if (jy) jy[k] = j;
+    if (pn) pn[k] = p;
+  }
+}
+
 // ------------------------------------------------------------------ winds
 // Conversion-factor tables (FP64), per panel:
 //   conv_pc [4][6][M][ld]      ex_lon, ex_lat, ey_lon, ey_lat at centres
@@ -1320,6 +1378,26 @@ int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *str
   KSum acc{0.0, 0.0};
   for (int b = 0; b < kRedBlocks; ++b) { kadd(acc, part[2 * b]); kadd(acc, part[2 * b + 1]); }
   out[0] = acc.s + acc.c;
+  return CSADV_OK;
+}
+
+int csadv_remap_latlon(csadv_t *h, int32_t solve, int32_t tracer, const double *field, int32_t nlat, int32_t nlon,
+                       double *out, int32_t *ix, int32_t *jy, int32_t *panel, void *stream) {
+  if (!h || !h->dev) return CSADV_E_STATE;
+  if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr)
+    return fail(h, CSADV_E_ARG, "remap: bad solve / tracer");
+  if (nlat < 2 || nlon < 2 || (int64_t)nlat * nlon > ((int64_t)1 << 31))
+    return fail(h, CSADV_E_RANGE, "remap: nlat, nlon must be >= 2 and nlat * nlon <= 2^31");
+  hipStream_t st = (hipStream_t)stream;
+  if (!field) hipLaunchKernelGGL(k_settle, dim3(1), dim3(BLOCK), 0, st, h->dev, solve);   // totals current
+  const int blocks = (int)std::min<int64_t>(4096, ((int64_t)nlat * nlon + BLOCK - 1) / BLOCK);
+  if (h->dtype == CSADV_F64)
+    hipLaunchKernelGGL(k_remap_latlon<double>, dim3(blocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, field, nlat,
+                       nlon, out, ix, jy, panel);
+  else
+    hipLaunchKernelGGL(k_remap_latlon<float>, dim3(blocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, field, nlat,
+                       nlon, out, ix, jy, panel);
+  HIPCHK(h, hipGetLastError());
   return CSADV_OK;
 }
 

@@ -17,7 +17,7 @@ _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of ord
 SYMBOLS = (
     "csadv_create", "csadv_partials_per_tracer", "csadv_work_bytes", "csadv_tile_rows", "csadv_update", "csadv_wind_init",
     "csadv_ghost_fill", "csadv_step", "csadv_set_graph", "csadv_finalize", "csadv_mass", "csadv_norms",
-    "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_time_step_kernel", "csadv_last_error", "csadv_destroy",
+    "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_time_step_kernel", "csadv_remap_latlon", "csadv_last_error", "csadv_destroy",
     "csadv_abi_version",
 )
 
@@ -78,6 +78,7 @@ def load():
     lib.csadv_profile.argtypes = [vp, i32]
     lib.csadv_profile_read.argtypes = [vp, dbl, ctypes.POINTER(ctypes.c_int32)]
     lib.csadv_time_step_kernel.argtypes = [vp, i32, vp, dbl]
+    lib.csadv_remap_latlon.argtypes = [vp, i32, i32, vp, i32, i32, vp, vp, vp, vp, vp]
     lib.csadv_last_error.argtypes = [vp]
     lib.csadv_last_error.restype = ctypes.c_char_p
     lib.csadv_destroy.argtypes = [vp]

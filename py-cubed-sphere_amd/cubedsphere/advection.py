@@ -9,13 +9,15 @@ src/advection_error.py:21-188, main.py test case 5) with the step on the GPU:
   update_adv(cs_grid, sim, t)            # src/advection_timestep.py:48  (no-op: see below)
   adv_sphere(cs_grid, sim) -> (linf, l1, l2)
   error_analysis_adv(sim) -> errors table of the convergence study (tc = 2), rates as the reference's
+  output_adv(cs_grid, sim, k, nsteps)    # src/output.py: errors, mass, lat-lon remaps, data files
   main(pardir)                           # test case 5 from par/configuration.par + advection.par
 
 Differences of contract, all deliberate: ``cs_grid`` is a :class:`cubedsphere.grid.Grid`;
 ``sim`` is a :class:`cubedsphere.solver.Advection` whose state lives on the GPU (read it
 with ``sim.state()``); the time-dependent winds of ``update_adv`` are recombined on the
-device at the start of the next step, so ``update_adv`` has nothing left to do; plotting
-and file output of ``output_adv`` are not part of this path.  Errors are those of
+device at the start of the next step, so ``update_adv`` has nothing left to do.  The output
+side (``output_adv``) computes the errors, mass, the device lat-lon remaps and the reference's
+data files; it draws no plots (cartopy is not available).  Errors are those of
 src/errors.py:99-113 at the final step.
 """
 import numpy as np
@@ -40,13 +42,22 @@ def adv_simulation_par(cs_grid, dt, Tf, ic, vf, tc, recon, dp, opsplit, et, mt, 
     sim = Advection(cs_grid.N, reference_case(ic, vf), dt, recon=recon, dp=dp, opsplit=opsplit, et=et, mt=mt,
                     mf=mf, dtype=dtype, device=device, tile=tile, max_steps=max(nsteps, 1), grid=cs_grid)
     sim.Tf, sim.tc, sim.ic, sim.vf = Tf, tc, ic, vf
+    sim.recon, sim.dp, sim.opsplit, sim.et, sim.mt, sim.mf = recon, dp, opsplit, et, mt, mf
     return sim
 
 
 def init_vars_adv(cs_grid, simulation):
-    """src/advection_vars.py:19-107: initial condition, winds, tables, the device handle."""
+    """src/advection_vars.py:19-107: initial condition, winds, tables, the device handle, and
+    the CFL number of :88-98 (the largest signed Courant number of the interior x-edges of all
+    rows and the interior y-edges of all columns, in absolute value)."""
     simulation.init_vars_adv()
     simulation.total_mass0 = simulation.mass()
+    M, i0, iend = cs_grid.M, cs_grid.i0, cs_grid.iend
+    U = simulation.usgn if simulation.usgn is not None else simulation.uavg
+    V = simulation.vsgn if simulation.vsgn is not None else simulation.vavg
+    cx = float((U[:, :M, i0:iend + 1].double() * simulation.dt / cs_grid.dx).max())
+    cy = float((V[:, i0:iend + 1, :M].double() * simulation.dt / cs_grid.dx).max())
+    simulation.CFL = max(abs(cx), abs(cy))
 
 
 def adv_time_step(cs_grid, simulation, k, t):
@@ -63,7 +74,7 @@ def update_adv(cs_grid, simulation, t):
     return None
 
 
-def adv_sphere(cs_grid, simulation, divtest_flag=False):
+def adv_sphere(cs_grid, simulation, divtest_flag=False, datadir=None, verbose=False):
     """src/advection_sphere.py:13-61 without plotting: run Tf/dt steps, return the final
     (L_inf, L_1, L_2) errors against the exact solution.  Nothing is read between steps
     (there is no per-step output here), so the steps go to the device in one call, which is
@@ -73,8 +84,75 @@ def adv_sphere(cs_grid, simulation, divtest_flag=False):
     if nsteps >= 64:
         simulation.use_graph(16)          # replay 16-step graphs: no per-launch host cost
     simulation.step(nsteps)
+    if datadir is not None or verbose:       # src/output.py at the final step
+        return output_adv(cs_grid, simulation, nsteps, nsteps, datadir=datadir, verbose=verbose)[0]
     simulation.mass_change = _mass_change(simulation.total_mass0, simulation.mass())
     return simulation.norms(t=nsteps * simulation.dt)
+
+
+# plotting grid of src/constants.py:36-37
+NLAT = 720
+NLON = 2 * NLAT
+
+RECON_NAMES = {1: "PPM-0", 2: "PPM-CW84", 3: "PPM-PL07", 4: "PPM-L04"}          # src/advection_ic.py:85-158
+DP_NAMES = {1: "RK1", 2: "RK2"}
+SPLIT_NAMES = {1: "SP-AVLT", 2: "SP-L04", 3: "SP-PL07"}
+ET_NAMES = {1: "ET-S72", 2: "ET-PL07", 3: "ET-DG"}
+MT_NAMES = {1: "MT-0", 2: "MT-PL07"}
+MF_NAMES = {1: "MF-0", 2: "MF-AF", 3: "MF-PR"}
+
+
+def ll2cs(cs_grid, simulation, nlat=NLAT, nlon=NLON):
+    """src/interpolation.py:38-139 on the device: the (i, j, panel) tables of the plotting grid,
+    each [nlon][nlat]."""
+    _, tabs = simulation.remap_latlon(nlat=nlat, nlon=nlon, tables=True)
+    return tabs
+
+
+def nearest_neighbour(simulation, field=None, tracer=0, nlat=NLAT, nlon=NLON):
+    """src/interpolation.py:144-149 on the device: the tracer's state (or an interior field
+    [6][N][N]) on the plotting grid, [nlon][nlat] FP64 numpy."""
+    return simulation.remap_latlon(tracer=tracer, nlat=nlat, nlon=nlon, field=field)
+
+
+def print_diagnostics_adv(error_linf, error_l1, error_l2, mass_change, t, Nsteps):
+    """src/output.py:11-14"""
+    print("\nStep", t, "from", Nsteps)
+    print("Error (Linf, L1, L2) :", "{:.2e}".format(error_linf), "{:.2e}".format(error_l1), "{:.2e}".format(error_l2))
+    print("Total mass variation:", "{:.2e}".format(mass_change))
+
+
+def output_name(simulation, error=True):
+    """File name stem of src/output.py:121-123 (error field) and :91-92 (state)."""
+    opts = (SPLIT_NAMES[simulation.opsplit], RECON_NAMES[simulation.recon], DP_NAMES[simulation.dp],
+            ET_NAMES[simulation.et], MT_NAMES[simulation.mt])
+    if error:
+        return ("adv_Q_error_ic%d_vf%d_%s_%s_%s_%s_%s_%s_interp3" % ((simulation.ic, simulation.vf) + opts +
+                                                                    (MF_NAMES[simulation.mf],)))
+    return "adv_Q_ic%d_vf%d_%s_%s_%s_%s_%s_interp3" % ((simulation.ic, simulation.vf) + opts)
+
+
+def output_adv(cs_grid, simulation, k, nsteps, datadir=None, nlat=NLAT, nlon=NLON, verbose=True):
+    """src/output.py:16-150 at step k, without plotting: errors against the exact solution,
+    total mass variation, the printout of step k > 0, and the lat-lon fields Q_ll and the
+    error Q_exact_ll - Q_ll (both remapped on the device).  With ``datadir`` at the final
+    step, writes the reference's files: <grid>_<name>.npy (the error field) and
+    <grid>_<name> (text: Linf, L1, L2, CFL, mass change, dt, Tf)."""
+    t = k * simulation.dt
+    linf, l1, l2 = simulation.norms(t=t)
+    simulation.mass_change = _mass_change(simulation.total_mass0, simulation.mass())
+    if verbose and k > 0:
+        print_diagnostics_adv(linf, l1, l2, simulation.mass_change, k, nsteps)
+    q_ll = nearest_neighbour(simulation, nlat=nlat, nlon=nlon)
+    qe_ll = nearest_neighbour(simulation, field=simulation.exact(0, t), nlat=nlat, nlon=nlon)
+    err_ll = qe_ll - q_ll
+    if datadir is not None and k == nsteps:
+        import os
+        stem = os.path.join(datadir, "%s_cs_%d_%s" % (cs_grid.projection, cs_grid.N, output_name(simulation)))
+        np.save(stem, err_ll, allow_pickle=False)
+        np.savetxt(stem, np.array([linf, l1, l2, getattr(simulation, "CFL", 0.0), simulation.mass_change,
+                                   simulation.dt, simulation.Tf]))
+    return (linf, l1, l2), q_ll, err_ll
 
 
 def _mass_change(m0, m):
@@ -133,7 +211,7 @@ def main(pardir="par"):
     g = Grid(gp.N)
     sim = adv_simulation_par(g, ap.dt, ap.Tf, ap.ic, ap.vf, ap.tc, ap.recon, ap.dp, ap.opsplit, ap.et, ap.mt, ap.mf)
     if ap.tc == 1:
-        return adv_sphere(g, sim)
+        return adv_sphere(g, sim, verbose=True)
     if ap.tc == 2:
         sim.close()
         return error_analysis_adv(sim, verbose=True)

@@ -328,6 +328,24 @@ class Advection:
         M = self.grid.M
         return self.q[self.current()][:, :, :, :M].cpu().numpy()
 
+    def remap_latlon(self, tracer=0, nlat=720, nlon=1440, field=None, tables=False):
+        """Nearest-cell remap onto the plotting lat-lon grid (src/interpolation.py:38-149):
+        [nlon][nlat] FP64 numpy of the tracer's state, or of ``field`` (an interior
+        [6][N][N] array, i fastest) when given; with ``tables`` also the (i, j, panel) index
+        tables of src/interpolation.py:38-139."""
+        dev = self.device
+        out = torch.empty((nlon, nlat), dtype=torch.float64, device=dev)
+        f = None
+        if field is not None:
+            f = torch.as_tensor(np.ascontiguousarray(field) if not torch.is_tensor(field) else field,
+                                dtype=torch.float64).to(dev).contiguous()
+        tabs = [torch.empty((nlon, nlat), dtype=torch.int32, device=dev) for _ in range(3)] if tables else [None] * 3
+        _lib.check(self.lib.csadv_remap_latlon(self.handle, self.index, int(tracer), _ptr(f), int(nlat), int(nlon),
+                                               _ptr(out), _ptr(tabs[0]), _ptr(tabs[1]), _ptr(tabs[2]), self._stream()),
+                   self.handle, "csadv_remap_latlon")
+        q = out.cpu().numpy()
+        return (q, tuple(t.cpu().numpy() for t in tabs)) if tables else q
+
     def mass(self, tracer=0):
         out = ctypes.c_double()
         _lib.check(self.lib.csadv_mass(self.handle, self.index, tracer, ctypes.byref(out), self._stream()), self.handle,

@@ -351,3 +351,52 @@ def test_vector_ghost_fill_converges_to_analytic_wind():
     errs = np.array(errs)
     assert errs[-1] < 1e-5
     assert np.all(errs[:-1] / errs[1:] > 7.0), errs          # third order or better
+
+
+# ------------------------------------------------------------------ lat-lon output remap
+class _Pts:
+    def __init__(self, X, Y, Z):
+        self.X, self.Y, self.Z = X, Y, Z
+
+
+def test_remap_sends_every_cell_centre_to_its_own_cell():
+    """A cell centre lies inside its cell, so ll2cs must return that cell and panel."""
+    from oracle import remap
+    N = 16
+    g = ogrid.CubedSphere(N)
+    I = slice(g.i0, g.iend)
+    i, j, p = remap.ll2cs(N, _Pts(g.pc.X[I, I, :], g.pc.Y[I, I, :], g.pc.Z[I, I, :]))
+    ii, jj, pp = np.meshgrid(np.arange(N), np.arange(N), np.arange(6), indexing="ij")
+    assert np.array_equal(i, ii) and np.array_equal(j, jj) and np.array_equal(p, pp)
+
+
+def test_remap_panel_is_the_face_the_point_projects_into_and_ties_go_to_the_later_face():
+    from oracle import remap
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(3, 4000))
+    v /= np.linalg.norm(v, axis=0)
+    pts = _Pts(v[0], v[1], v[2])
+    _, _, p = remap.ll2cs(8, pts)
+    for q in range(6):
+        m = p == q
+        x, y = ogrid.inverse_equiangular_gnomonic_map(v[0][m], v[1][m], v[2][m], q)
+        assert np.all(abs(x) <= ogrid.PIO4 * (1 + 1e-12)) and np.all(abs(y) <= ogrid.PIO4 * (1 + 1e-12))
+    # ties: |X| = |Y| with X, Y > 0 goes to panel 1; the cube corner X = Y = Z > 0 to panel 4
+    s = 1.0 / math.sqrt(2.0)
+    c = 1.0 / math.sqrt(3.0)
+    _, _, p = remap.ll2cs(8, _Pts(np.array([s, c, -s]), np.array([s, c, -s]), np.array([0.0, c, 0.0])))
+    assert list(p) == [1, 4, 3]
+
+
+def test_remap_of_a_smooth_field_converges_first_order():
+    """Nearest-cell sampling of q = sin(lat) errs by at most about half a cell diagonal."""
+    from oracle import remap
+    ll = remap.LatLonGrid(90, 180)
+    errs = []
+    for N in (16, 32):
+        g = ogrid.CubedSphere(N)
+        I = slice(g.i0, g.iend)
+        i, j, p = remap.ll2cs(N, ll)
+        q_ll = remap.nearest_neighbour(np.sin(g.pc.lat[I, I, :]), i, j, p)
+        errs.append(np.abs(q_ll - np.sin(ll.lat)).max())
+    assert errs[0] < 1.2 * (math.pi / 2 / 16) and 1.6 < errs[0] / errs[1] < 2.5, errs
