@@ -261,7 +261,7 @@ __global__ void k_settle(Solve *solves, int first) {
 // The windows are rings indexed by the iteration's phase modulo 6 (the loop is unrolled by
 // six, so every slot is a fixed register and nothing is shifted); row quantities that the
 // column stage needs four iterations later (metric, 1/g, wind) ride in the same rings.
-// The inputs of row y+2 are requested at the end of iteration y into one of two rotating
+// The inputs of row y+PF are requested at the end of iteration y into one of PF = 2 rotating
 // buffers, so their memory latency overlaps a whole iteration of arithmetic.
 // (src/discrete_operators.py:18-101, src/flux.py, src/reconstruction_1d.py,
 // src/advection_timestep.py:43)
@@ -277,6 +277,9 @@ struct StepIn {     // inputs of iteration y at the lane's column: row y and y-e
 template <typename T, int RECON, bool RK2, int SPLIT>
 struct March {
   static constexpr int split = SPLIT;
+  // rows whose inputs are in flight ahead of the one being processed (a third row measured no
+  // faster at N=384 FP64 and costs 18-28 VGPRs)
+  static constexpr int PF = 2;
   static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
   const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
@@ -394,7 +397,7 @@ struct March {
     // row y replaces row y-6 in every ring
     qw[s0] = q; xw[s0] = qx; gcw[s0] = gc0; rgw[s0] = in.rg; uw[s0] = u; usw[s0] = in.us; guw[s0] = gu0;
     if constexpr (std::is_same<T, float>::value) sw[s0] = in.raw;
-    if (y + 2 <= ylast) load(y + 2, in);
+    if (y + PF <= ylast) load(y + PF, in);
   }
 
   __device__ __forceinline__ void run() {
@@ -404,20 +407,20 @@ struct March {
     g1prev = g2prev = vprev = T(0);
     pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
     part = 0.0;
-    StepIn<T> b0, b1;
-    const int y0 = tj - 3;
-    load(y0, b0);
-    load(y0 + 1, b1);
+    StepIn<T> b[PF];
+    const int y0 = tj - 3;            // ylast - y0 = ny + 6 >= PF: every initial row exists
+#pragma unroll
+    for (int k = 0; k < PF; ++k) load(y0 + k, b[k]);
     int y = y0;
     for (; y + 5 <= ylast; y += 6) {
-      body<0>(y, b0); body<1>(y + 1, b1); body<2>(y + 2, b0);
-      body<3>(y + 3, b1); body<4>(y + 4, b0); body<5>(y + 5, b1);
+      body<0>(y, b[0]); body<1>(y + 1, b[1 % PF]); body<2>(y + 2, b[2 % PF]);
+      body<3>(y + 3, b[3 % PF]); body<4>(y + 4, b[4 % PF]); body<5>(y + 5, b[5 % PF]);
     }
-    if (y <= ylast) body<0>(y, b0);
-    if (y + 1 <= ylast) body<1>(y + 1, b1);
-    if (y + 2 <= ylast) body<2>(y + 2, b0);
-    if (y + 3 <= ylast) body<3>(y + 3, b1);
-    if (y + 4 <= ylast) body<4>(y + 4, b0);
+    if (y <= ylast) body<0>(y, b[0]);
+    if (y + 1 <= ylast) body<1>(y + 1, b[1 % PF]);
+    if (y + 2 <= ylast) body<2>(y + 2, b[2 % PF]);
+    if (y + 3 <= ylast) body<3>(y + 3, b[3 % PF]);
+    if (y + 4 <= ylast) body<4>(y + 4, b[4 % PF]);
   }
 };
 
