@@ -283,7 +283,6 @@ struct March {
   static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
   const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
-  const double *gc64;
   T *Qo;
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
   bool out_lane, in_col, need_col, mfpr;
@@ -305,8 +304,16 @@ struct March {
     r.gu = ld_at(gu, ou);
     r.gc = ld_at(gc, o);
     r.rg = ld_at(rgc, o);
-    r.q = T(0);    // rows outside the stencils' reach
-    if (need_col && y >= 1 && y < iend + 3) r.q = ld_at(Q, (uint32_t)(y * ld + x));
+    if constexpr (std::is_same<T, float>::value) {
+      // FP32: Q of every lane is read (clamped column; rows y0..ylast lie in 1..M-1) and zeroed
+      // outside the stencils' reach by a select, with no branch around the load (measured
+      // faster for FP32, slower for FP64, whose registers it raises)
+      const T qv = ld_at(Q, (uint32_t)(y * ld + xc));
+      r.q = (need_col && y >= 1 && y < iend + 3) ? qv : T(0);
+    } else {
+      r.q = T(0);    // rows outside the stencils' reach
+      if (need_col && y >= 1 && y < iend + 3) r.q = ld_at(Q, (uint32_t)(y * ld + x));
+    }
     const int ec = min(max(y - 3, 0), M - 1);
     r.v = ld_at(V, (uint32_t)(ec * ld + xc));
     r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
@@ -314,11 +321,18 @@ struct March {
   }
 
   // interior cells carry the mass-fixer correction total (ghosts got theirs in k_ghost)
+  // (Q = S + C g in the solve's precision: C g is ~1e-6 of q at most, so its FP32 rounding is
+  // far below that of the sum, which is rounded once)
   __device__ __forceinline__ void correct(int y, StepIn<T> &r) const {
     r.raw = r.q;
-    if (cf != 0.0 && in_col && y >= NG && y < iend) {
-      const double g = std::is_same<T, double>::value ? (double)r.gc : ld_at(gc64, (uint32_t)(y * ld + x));
-      r.q = (T)((double)r.q + cf * g);
+    if constexpr (std::is_same<T, float>::value) {
+      if (cf != 0.0) {
+        const bool interior = in_col && y >= NG && y < iend;
+        const T c = r.q + T(cf) * r.gc;
+        r.q = interior ? c : r.q;
+      }
+    } else {
+      if (cf != 0.0 && in_col && y >= NG && y < iend) r.q = r.q + cf * r.gc;
     }
   }
 
@@ -469,7 +483,6 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
   m.gu = reinterpret_cast<const T *>(D.gu);
   m.gv = reinterpret_cast<const T *>(D.gv);
   m.rgc = reinterpret_cast<const T *>(D.rgc);
-  m.gc64 = D.gc64;
   const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
   T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
   m.Q = Qin + tr * S.tstride + p * S.pstride;
