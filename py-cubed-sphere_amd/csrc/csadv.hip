@@ -354,11 +354,11 @@ struct March {
         const T v = in.v;
         const bool vup = in.vs >= T(0);
         const T cy = v * dtdy;
-        const T g1 = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, v, gv0, mt0);
-        const T g2 = upwind<T>(vup, pb_A, pb_B, pb_q6, b, cy, v, gv0, mt0);
+        const T g1 = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, gv0, mt0);     // times dt/dy
+        const T g2 = upwind<T>(vup, pb_A, pb_B, pb_q6, b, cy, gv0, mt0);
         if (y >= tj + 4) {
           const T gcm = gcw[s2];               // row y-4
-          const T dG1 = (g1prev - g1) * dtdy;
+          const T dG1 = g1prev - g1;
           T d1 = T(0), d2 = T(0);
           if (split != 1) {
             d1 = gv0 * cy;
@@ -372,9 +372,9 @@ struct March {
           const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
           const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
           const Parab<T> c = parabola<T, RECON>(qy_m2, qy_m1, qy, qy_p1, qy_p2, gur, from_right(gur), gcm, mt0);
-          const T f2 = upwind<T>(urp, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, ur * dtdx, ur, gur, mt0);
-          const T dF2 = (f2 - from_right(f2)) * dtdx;
-          const T dG2 = (g2prev - g2) * dtdy;
+          const T f2 = upwind<T>(urp, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, ur * dtdx, gur, mt0);
+          const T dF2 = f2 - from_right(f2);
+          const T dG2 = g2prev - g2;
           // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
           const T flx = dF2 + dG2;
           if (out_lane && y <= ylast) {
@@ -400,8 +400,8 @@ struct March {
     const T q_m1 = from_left(q), q_p1 = from_right(q);
     const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
     const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
-    const T f1 = upwind<T>(up, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, u * dtdx, u, gu0, mt0);
-    const T dF1 = (f1 - from_right(f1)) * dtdx;
+    const T f1 = upwind<T>(up, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, u * dtdx, gu0, mt0);
+    const T dF1 = f1 - from_right(f1);
     T c1 = T(0), c2 = T(0);
     if (split != 1) {
       c1 = gup1 * (from_right(u) * dtdx);

@@ -133,12 +133,15 @@ __device__ __forceinline__ Parab<T> parabola(T m2, T m1, T c0, T p1, T p2, T gl,
 // Upwind PPM flux through one edge (src/flux.py:48-70) from the parabolas of the cells on its
 // left (L) and right (R): f = q_R + (c/2)(q6 - dq) - q6 c^2/3 from the left cell when up,
 // f = q_L - (c/2)(q6 + dq) - q6 c^2/3 from the right cell otherwise; F = f u_avg, and
-// MT-PL07 multiplies by the edge metric ge.
+// MT-PL07 multiplies by the edge metric ge.  Returned already times dt/dx, as F dt/dx =
+// f c (c = u_avg dt/dx, the Courant number), so the flux difference of a cell is the
+// difference of two returned values; with h = c/2, f = A + h (B - (4/3) q6 h), which takes
+// five operations where the textbook order takes seven.
 template <typename T>
-__device__ __forceinline__ T upwind(bool up, T LrA, T LrB, T Lq6, const Parab<T> &R, T cc, T uav, T ge, bool mt0) {
+__device__ __forceinline__ T upwind(bool up, T LrA, T LrB, T Lq6, const Parab<T> &R, T cc, T ge, bool mt0) {
   const T A = up ? LrA : R.lA, B = up ? LrB : -R.lB, q6 = up ? Lq6 : R.q6;
-  T f = A + (cc * T(0.5)) * B - ((q6 * cc) * cc) * T(1.0 / 3.0);
-  f = f * uav;
+  const T h = cc * T(0.5);
+  T f = (A + h * (B - T(4.0 / 3.0) * (q6 * h))) * cc;
   if (!mt0) f = f * ge;
   return f;
 }
