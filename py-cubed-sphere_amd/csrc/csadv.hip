@@ -108,7 +108,7 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
 // weighted sum of interior cells of other panels (edge strips: 4 sources; corners: 16, the
 // two interpolation levels of src/interpolation.py:250-314 multiplied out).  Interior
 // sources carry the pending mass-fixer correction: sum_l w_l (Q_l + c g_l) = sum_l w_l Q_l + c gw.
-// grid: x = recipe cells (strips then corners), y = solve.
+// grid: x = recipe cells (strips then corners), y = solve, z = tracer.
 // Control words of a solve (device int32 ctl[4], zero at init), each written by one kernel
 // only and read by the other, so no kernel reads a word it writes:
 //   ctl[1]  steps completed           written by k_step (one lane), read by k_ghost, k_wind_avg
@@ -163,6 +163,8 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
   __shared__ double red[BLOCK];
   const Solve &S = solves[blockIdx.y];
   const csadv_solve_desc &D = S.d;
+  const int tr = blockIdx.z;           // one tracer per grid layer: tracers run side by side
+  if (tr >= D.ntr) return;
   const int ncell = D.n_strip + D.n_corner;
   if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
   const int idx = blockIdx.x * BLOCK + threadIdx.x;
@@ -201,38 +203,34 @@ __global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
     }
   }
   const int steps = D.ctl[1], pending = D.ctl[3];
+  const bool lead = blockIdx.x == 0 && tr == 0 && threadIdx.x == 0;
   if (D.stop > 0 && steps >= D.stop) {      // solve finished: k_step reads ctl[2] and skips it too
-    if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
+    if (lead) D.ctl[2] = steps;
     return;
   }
-  T *Q = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0);
-  for (int tr = 0; tr < D.ntr; ++tr) {
-    T *Qt = Q + tr * S.tstride;
-    // weighted sum of the sources first (in recipe order), so that its loads overlap the
-    // block-wide reduction of the mass-fixer partials below
-    double acc = 0.0;
-    if (active) {
-      if (strip) {
-        const T v0 = Qt[s16[0]], v1 = Qt[s16[1]], v2 = Qt[s16[2]], v3 = Qt[s16[3]];
-        acc += (double)v0 * w16[0];
-        acc += (double)v1 * w16[1];
-        acc += (double)v2 * w16[2];
-        acc += (double)v3 * w16[3];
-      } else {
-        T v[16];
+  T *Qt = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0) + tr * S.tstride;
+  // weighted sum of the sources first (in recipe order), so that its loads overlap the
+  // block-wide reduction of the mass-fixer partials below
+  double acc = 0.0;
+  if (active) {
+    if (strip) {
+      const T v0 = Qt[s16[0]], v1 = Qt[s16[1]], v2 = Qt[s16[2]], v3 = Qt[s16[3]];
+      acc += (double)v0 * w16[0];
+      acc += (double)v1 * w16[1];
+      acc += (double)v2 * w16[2];
+      acc += (double)v3 * w16[3];
+    } else {
+      T v[16];
 #pragma unroll
-        for (int l = 0; l < 16; ++l) v[l] = Qt[s16[l]];
+      for (int l = 0; l < 16; ++l) v[l] = Qt[s16[l]];
 #pragma unroll
-        for (int l = 0; l < 16; ++l) acc += (double)v[l] * w16[l];
-      }
+      for (int l = 0; l < 16; ++l) acc += (double)v[l] * w16[l];
     }
-    const double cf = pending_corr(S, tr, red, pending, steps);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && pending) D.corr[(steps & 1) * D.ntr + tr] = cf;
-    if (!active) continue;
-    acc += cf * gw;
-    Qt[tgt] = (T)acc;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) D.ctl[2] = steps;
+  const double cf = pending_corr(S, tr, red, pending, steps);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && pending) D.corr[(steps & 1) * D.ntr + tr] = cf;
+  if (active) Qt[tgt] = (T)(acc + cf * gw);
+  if (lead) D.ctl[2] = steps;
 }
 
 // Add the last step's partials to the correction total (once; used by finalize and the
@@ -716,7 +714,10 @@ __global__ void k_wind_avg(Solve *solves) {
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < 2 * ne * M; idx += gridDim.x * blockDim.x) {
     const bool xdir = idx < ne * M;
     const int id2 = xdir ? idx : idx - ne * M;
-    const int e = id2 % ne + i0, c = id2 / ne;      // edge index along the flux direction, row/col
+    // consecutive threads take consecutive addresses: the edge index runs fastest on x-edges
+    // ([row][edge]), the column on y-edges ([edge][column])
+    const int e = (xdir ? id2 % ne : id2 / M) + i0;   // edge index along the flux direction
+    const int c = xdir ? id2 / ne : id2 % M;          // row (x-edges) / column (y-edges)
     const int64_t bs = xdir ? bs_u : bs_v;
     const double *W = xdir ? Wu : Wv;
     const int64_t off = xdir ? (int64_t)c * ldu + e : (int64_t)e * ld + c;
@@ -962,7 +963,9 @@ int step_ghosts(csadv_t *h, hipStream_t st) {
   const int ns = (int)h->desc.size();
   int maxc = 0;
   for (auto &d : h->desc) maxc = std::max(maxc, d.n_strip + d.n_corner);
-  const dim3 grid((maxc + BLOCK - 1) / BLOCK, ns);
+  int maxtr = 1;
+  for (auto &d : h->desc) maxtr = std::max(maxtr, d.ntr);
+  const dim3 grid((maxc + BLOCK - 1) / BLOCK, ns, maxtr);
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_ghost<double>, grid, dim3(BLOCK), 0, st, h->dev);
   else
