@@ -159,9 +159,7 @@ __device__ double pending_corr(const Solve &S, int tr, double *red, int pending,
 }
 
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
-  __shared__ double red[BLOCK];
-  const Solve &S = solves[blockIdx.y];
+__device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
   const csadv_solve_desc &D = S.d;
   const int tr = blockIdx.z;           // one tracer per grid layer: tracers run side by side
   if (tr >= D.ntr) return;
@@ -247,6 +245,19 @@ __global__ void k_settle(Solve *solves, int first) {
   }
   __syncthreads();
   if (threadIdx.x == 0) D.ctl[3] = 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
+  __shared__ double red[BLOCK];
+  ghost_cells<T>(solves[blockIdx.y], red);
+}
+
+// A handle of one solve: the descriptor travels in the kernel arguments.
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
+  __shared__ double red[BLOCK];
+  ghost_cells<T>(one, red);
 }
 
 // Work item = (solve, panel, tracer, strip, segment): one wavefront marches down strip_w interior
@@ -437,11 +448,8 @@ struct March {
 };
 
 template <typename T, int RECON, bool RK2, int SPLIT>
-__global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *work) {
+__device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int lane = threadIdx.x;
-  const int4 wk = work[blockIdx.x];
-  if (wk.x < 0) return;              // padding item of the XCD-banded work list
-  const Solve &S = solves[wk.x];
   const csadv_solve_desc &D = S.d;
   const int p = wk.y & 7, tr = wk.y >> 3;    // panel, tracer
   const int N = D.N;
@@ -491,6 +499,22 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *solves, const int4 *
     const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
     if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
   }
+}
+
+template <typename T, int RECON, bool RK2, int SPLIT>
+__global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves, const int4 *__restrict__ work) {
+  const int4 wk = work[blockIdx.x];
+  if (wk.x < 0) return;              // padding item of the XCD-banded work list
+  step_item<T, RECON, RK2, SPLIT>(solves[wk.x], wk);
+}
+
+// A handle of one solve: the descriptor travels in the kernel arguments, so the work item and
+// the step counter are requested at once instead of after a descriptor load.
+template <typename T, int RECON, bool RK2, int SPLIT>
+__global__ __launch_bounds__(WAVE) void k_step1(const Solve one, const int4 *work) {
+  const int4 wk = work[blockIdx.x];
+  if (wk.x < 0) return;
+  step_item<T, RECON, RK2, SPLIT>(one, wk);
 }
 
 // Fold the correction totals into the current buffer (interior only); k_clear_corr then
@@ -928,10 +952,14 @@ Solve derive(const csadv_solve_desc &d, int tile) {
 
 template <typename T, int R, int SP>
 hipError_t launch_step_s(csadv_t *h, hipStream_t st) {
-  if (h->rk2)
-    hipLaunchKernelGGL((k_step<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
-  else
-    hipLaunchKernelGGL((k_step<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  const bool one = h->host.size() == 1;
+  if (h->rk2) {
+    if (one) hipLaunchKernelGGL((k_step1<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0], h->work);
+    else hipLaunchKernelGGL((k_step<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  } else {
+    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0], h->work);
+    else hipLaunchKernelGGL((k_step<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+  }
   return hipGetLastError();
 }
 
@@ -966,10 +994,14 @@ int step_ghosts(csadv_t *h, hipStream_t st) {
   int maxtr = 1;
   for (auto &d : h->desc) maxtr = std::max(maxtr, d.ntr);
   const dim3 grid((maxc + BLOCK - 1) / BLOCK, ns, maxtr);
-  if (h->dtype == CSADV_F64)
-    hipLaunchKernelGGL(k_ghost<double>, grid, dim3(BLOCK), 0, st, h->dev);
-  else
-    hipLaunchKernelGGL(k_ghost<float>, grid, dim3(BLOCK), 0, st, h->dev);
+  const bool one = ns == 1;
+  if (h->dtype == CSADV_F64) {
+    if (one) hipLaunchKernelGGL(k_ghost1<double>, grid, dim3(BLOCK), 0, st, h->host[0]);
+    else hipLaunchKernelGGL(k_ghost<double>, grid, dim3(BLOCK), 0, st, h->dev);
+  } else {
+    if (one) hipLaunchKernelGGL(k_ghost1<float>, grid, dim3(BLOCK), 0, st, h->host[0]);
+    else hipLaunchKernelGGL(k_ghost<float>, grid, dim3(BLOCK), 0, st, h->dev);
+  }
   HIPCHK(h, hipGetLastError());
   return CSADV_OK;
 }
