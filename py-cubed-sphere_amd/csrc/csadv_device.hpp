@@ -146,11 +146,6 @@ __device__ __forceinline__ T upwind(bool up, T LrA, T LrB, T Lq6, const Parab<T>
   return f;
 }
 
-template <typename T>
-__device__ __forceinline__ T lane_shfl(T v, int src) {
-  return __shfl(v, src, 64);
-}
-
 // Whole-wavefront DPP lane shifts (one v_mov_b32_dpp per dword, no LDS round trip):
 // from_left(v) is v of lane-1, from_right(v) is v of lane+1; the end lane without a source
 // (lane 0, resp. lane 63) reads zero.  The step kernel only ever stores results of lanes
