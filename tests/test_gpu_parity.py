@@ -126,6 +126,29 @@ def test_two_hills_tilted_n384():
     assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
 
 
+def test_two_hills_tilted_n384_after_1_10_and_1000_steps():
+    """The N=384 FP64 configuration at the checkpoints of its parity specification: the device
+    state within 1e-12 (relative max) of the oracle after 1, 10 and 1000 steps (about 23% of a
+    revolution; the oracle takes about a minute on one core)."""
+    from cubedsphere.cases import workload
+    from cubedsphere.grid import Grid
+    from cubedsphere.solver import cfl_dt
+    N = 384
+    case = workload("two_hills_tilted")
+    dt = cfl_dt(Grid(N), case, 0.5)
+    sol = _solver(N, case, dt)
+    grid, sim = ots.make(N, oracle_case(case), dt)
+    ots.init_vars_adv(grid, sim)
+    k = 0
+    for target in (1, 10, 1000):
+        sol.step(target - k)
+        while k < target:
+            k += 1
+            ots.adv_time_step(grid, sim, k, k * dt)
+            ots.update_adv(grid, sim, k * dt)
+        assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64, target
+
+
 def test_mass_and_norms_match_oracle():
     N, dt, nsteps = 32, 0.01, 20
     case = _ref(2, 1)
