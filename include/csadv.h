@@ -15,7 +15,7 @@
  *   tracer state  q[b] : [ntr][6][M][ld]     i (x) fastest, j (y), panel, tracer
  *   x-edge winds       : [6][M][ldu]         e = 0..M along x, ldu >= M + 1
  *   y-edge winds       : [6][M + 1][ld]      e = 0..M along y
- *   metric  gc / rgc   : [M][ld]   gu : [M][ldu]   gv : [M + 1][ld]  (panel invariant)
+ *   metric  gc / rgc / gfix : [M][ld]   gu : [M][ldu]   gv : [M + 1][ld]  (panel invariant)
  * The reference stores the same values as numpy arrays indexed [i, j, panel].
  */
 #ifndef CSADV_H
@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CSADV_ABI_VERSION 4
+#define CSADV_ABI_VERSION 5
 
 enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
 
@@ -64,6 +64,11 @@ typedef struct csadv_solve_desc {
   void *uavg, *vavg;               /* time-averaged contravariant winds (dtype)         */
   void *usgn, *vsgn;               /* contravariant winds fixing the upwind side (dtype)*/
   const void *gc, *gu, *gv, *rgc;  /* metric tables (dtype)                             */
+  const void *gfix;                /* MF-PR: the metric the correction total multiplies
+                                      (dtype): gc inside the panel, sum_l w_l gc(src_l) of
+                                      the ghost recipe at the 3 ghost layers (the same on
+                                      every panel); the per-step ghost fill stores the
+                                      uncorrected sums and the step adds C gfix on read     */
   const double *gc64;              /* metric at centres, FP64 (mass fixer, mass)        */
   const double *lagw;              /* Lagrange weights [4][M][4], FP64                  */
   const int32_t *lagk;             /* stencil starts   [4][M]                            */
@@ -77,13 +82,11 @@ typedef struct csadv_solve_desc {
   const int32_t *strip_tgt;        /* [n_strip] target offsets inside one tracer           */
   const int32_t *strip_src;        /* [n_strip][4] source offsets inside one tracer        */
   const double *strip_w;           /* [n_strip][4] weights                                  */
-  const double *strip_gw;          /* [n_strip] sum_l w_l g(src_l): mass-fixer correction   */
   const int32_t *corner_tgt;       /* [n_corner]                                            */
   const int32_t *corner_src;       /* [n_corner][16]                                        */
   const double *corner_w;          /* [n_corner][16]                                        */
-  const double *corner_gw;         /* [n_corner]                                            */
   const double *coef;              /* [max_steps][2][nbasis] basis coefficients (t_{n-1}, t_{n-2}) */
-  double *corr;                    /* [2][ntr] MF-PR correction totals C (FP64), slot by step parity: Q = S + C g */
+  double *corr;                    /* [2][ntr] MF-PR correction totals C (FP64), slot by step parity: Q = S + C gfix */
   double *partials;                /* scratch, >= ntr * csadv_partials_per_tracer()     */
   int32_t *ctl;                    /* [4] device control words: cur, step, 0, 0          */
   int32_t tile;                    /* step tile shape of this solve (csadv_tile_rows);

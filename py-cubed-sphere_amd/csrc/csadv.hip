@@ -106,9 +106,13 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
 // ------------------------------------------------------------------ step ghost fill
 // Ghost cells a step reads (3 layers) from flattened duo-grid recipes: every value is a
 // weighted sum of interior cells of other panels (edge strips: 4 sources; corners: 16, the
-// two interpolation levels of src/interpolation.py:250-314 multiplied out).  Interior
-// sources carry the pending mass-fixer correction: sum_l w_l (Q_l + c g_l) = sum_l w_l Q_l + c gw.
-// grid: x = recipe cells (strips then corners), y = solve, z = tracer.
+// two interpolation levels of src/interpolation.py:250-314 multiplied out).  The sum is taken
+// over the stored values S; the step adds the correction total to every cell it reads as
+// C gfix, where gfix is g inside the panel and sum_l w_l g_l at a ghost cell (the same table
+// for the six panels), so that sum_l w_l (S_l + C g_l) = sum_l w_l S_l + C gfix.  The ghost
+// values therefore never wait for the correction total, which one more block per
+// (solve, tracer) reduces beside them.
+// grid: x = recipe blocks + 1 (the last: correction total), y = solve, z = tracer.
 // Control words of a solve (device int32 ctl[4], zero at init), each written by one kernel
 // only and read by the other, so no kernel reads a word it writes:
 //   ctl[1]  steps completed           written by k_step (one lane), read by k_ghost, k_wind_avg
@@ -119,14 +123,15 @@ __global__ void k_ghost_corners(const FieldSet *sets) {
 // the mass-fixer corrections, kept in corr[(n & 1) * ntr + tracer] for step n.  Folding a
 // correction of ~1e-9 into an FP32 state at every step would round it away where |S| is
 // large and keep it where S is small, a bias that grows with the step count.
+// Sum over a block of BLOCK threads in a fixed order (within each wavefront by halving
+// distances, then the wavefronts in index order); every thread gets the total.
 __device__ __forceinline__ double block_sum_fixed(double v, double *red) {
-  red[threadIdx.x] = v;
+  v = warp_sum(v);
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = v;
   __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  const double s = red[0];
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / WAVE; ++w) s += red[w];
   __syncthreads();
   return s;
 }
@@ -163,6 +168,14 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
   const csadv_solve_desc &D = S.d;
   const int tr = blockIdx.z;           // one tracer per grid layer: tracers run side by side
   if (tr >= D.ntr) return;
+  if (blockIdx.x == gridDim.x - 1) {   // the correction total after the last step
+    if (D.mf != 3) return;
+    const int steps = D.ctl[1], pending = D.ctl[3];
+    if (D.stop > 0 && steps >= D.stop) return;
+    const double cf = pending_corr(S, tr, red, pending, steps);
+    if (threadIdx.x == 0 && pending) D.corr[(steps & 1) * D.ntr + tr] = cf;
+    return;
+  }
   const int ncell = D.n_strip + D.n_corner;
   if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
   const int idx = blockIdx.x * BLOCK + threadIdx.x;
@@ -170,11 +183,10 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
   const bool strip = idx < D.n_strip;
   const int c = strip ? idx : idx - D.n_strip;
   // the recipe of this cell does not depend on the step: request all of it first, so that
-  // its latency overlaps the read of the step counter and the partial sums below
+  // its latency overlaps the read of the step counter
   int s16[16];
   double w16[16];
   int tgt = 0;
-  double gw = 0.0;
   if (active) {
     if (strip) {
       const int4 s4 = reinterpret_cast<const int4 *>(D.strip_src)[c];
@@ -183,7 +195,6 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
       const double2 a = w2[0], b = w2[1];
       w16[0] = a.x; w16[1] = a.y; w16[2] = b.x; w16[3] = b.y;
       tgt = D.strip_tgt[c];
-      gw = D.strip_gw[c];
     } else {
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
@@ -197,19 +208,16 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
         w16[2 * l] = a.x; w16[2 * l + 1] = a.y;
       }
       tgt = D.corner_tgt[c];
-      gw = D.corner_gw[c];
     }
   }
-  const int steps = D.ctl[1], pending = D.ctl[3];
+  const int steps = D.ctl[1];
   const bool lead = blockIdx.x == 0 && tr == 0 && threadIdx.x == 0;
   if (D.stop > 0 && steps >= D.stop) {      // solve finished: k_step reads ctl[2] and skips it too
     if (lead) D.ctl[2] = steps;
     return;
   }
   T *Qt = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0) + tr * S.tstride;
-  // weighted sum of the sources first (in recipe order), so that its loads overlap the
-  // block-wide reduction of the mass-fixer partials below
-  double acc = 0.0;
+  double acc = 0.0;                         // in recipe order
   if (active) {
     if (strip) {
       const T v0 = Qt[s16[0]], v1 = Qt[s16[1]], v2 = Qt[s16[2]], v3 = Qt[s16[3]];
@@ -224,10 +232,8 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
 #pragma unroll
       for (int l = 0; l < 16; ++l) acc += (double)v[l] * w16[l];
     }
+    Qt[tgt] = (T)acc;
   }
-  const double cf = pending_corr(S, tr, red, pending, steps);
-  if (blockIdx.x == 0 && threadIdx.x == 0 && D.mf == 3 && pending) D.corr[(steps & 1) * D.ntr + tr] = cf;
-  if (active) Qt[tgt] = (T)(acc + cf * gw);
   if (lead) D.ctl[2] = steps;
 }
 
@@ -277,6 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
 template <typename T>
 struct StepIn {     // inputs of iteration y at the lane's column: row y and y-edge y-3
   T q, u, us, gu, gc, rg;    // row y: Q, x-edge wind, upwind-side wind, metric, 1/g
+  T gf;                      // row y: metric the correction total multiplies (gfix)
   T raw;                     // row y: the stored value before the correction total is added
   T v, vs, gvp;              // y-edge y-3: wind, upwind-side wind; metric of y-edge y-2
 };
@@ -291,10 +298,10 @@ struct March {
   static constexpr int PF = 2;
   static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
-  const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
+  const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
   T *Qo;
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
-  bool out_lane, in_col, need_col, mfpr;
+  bool out_lane, need_col, mfpr, corrects;
   T dtdx, dtdy;
   double cf;
   // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6)
@@ -313,6 +320,7 @@ struct March {
     r.gu = ld_at(gu, ou);
     r.gc = ld_at(gc, o);
     r.rg = ld_at(rgc, o);
+    if (corrects) r.gf = ld_at(gfix, o);
     if constexpr (std::is_same<T, float>::value) {
       // FP32: Q of every lane is read (clamped column; rows y0..ylast lie in 1..M-1) and zeroed
       // outside the stencils' reach by a select, with no branch around the load (measured
@@ -329,27 +337,21 @@ struct March {
     r.gvp = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
   }
 
-  // interior cells carry the mass-fixer correction total (ghosts got theirs in k_ghost)
-  // (Q = S + C g in the solve's precision: C g is ~1e-6 of q at most, so its FP32 rounding is
-  // far below that of the sum, which is rounded once)
-  __device__ __forceinline__ void correct(int y, StepIn<T> &r) const {
+  // every cell the stencils read carries the mass-fixer correction total C: Q = S + C gfix,
+  // gfix being g inside the panel and, at a ghost cell, the weighted metric of the cells its
+  // value was interpolated from (in the solve's precision: C gfix is ~1e-6 of q at most, so
+  // its FP32 rounding is far below that of the sum, which is rounded once).  Lanes and rows
+  // outside the stencils' reach hold 0 + C gfix, which no stored cell reads.
+  __device__ __forceinline__ void correct(StepIn<T> &r) const {
     r.raw = r.q;
-    if constexpr (std::is_same<T, float>::value) {
-      if (cf != 0.0) {
-        const bool interior = in_col && y >= NG && y < iend;
-        const T c = r.q + T(cf) * r.gc;
-        r.q = interior ? c : r.q;
-      }
-    } else {
-      if (cf != 0.0 && in_col && y >= NG && y < iend) r.q = r.q + cf * r.gc;
-    }
+    if (corrects) r.q = r.q + T(cf) * r.gf;
   }
 
   template <int K>
   __device__ __forceinline__ void body(const int y, StepIn<T> &in) {
     constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
                   s5 = (K + 5) % 6;
-    correct(y, in);
+    correct(in);
     gvm1 = gv0;
     gv0 = gvp1;
     gvp1 = in.gvp;
@@ -475,7 +477,6 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.tj = tj;
   m.ylast = tj + ny + 3;
   m.out_lane = lane >= 3 && lane < 3 + nx;
-  m.in_col = m.x >= NG && m.x < iend;
   m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
   m.mfpr = mfpr;
   m.dtdx = T(D.dt / D.dx);
@@ -489,11 +490,13 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.gu = reinterpret_cast<const T *>(D.gu);
   m.gv = reinterpret_cast<const T *>(D.gv);
   m.rgc = reinterpret_cast<const T *>(D.rgc);
+  m.gfix = reinterpret_cast<const T *>(D.gfix);
   const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
   T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
   m.Q = Qin + tr * S.tstride + p * S.pstride;
   m.Qo = Qout + tr * S.tstride + p * S.pstride;
   m.cf = D.corr ? D.corr[(step & 1) * D.ntr + tr] : 0.0;
+  m.corrects = m.cf != 0.0;
   m.run();
   if (mfpr) {
     const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
@@ -920,9 +923,9 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     return CSADV_E_ARG;
   }
   if (d.dp == 2 && (!d.usgn || !d.vsgn)) { msg = "dp=2 needs the usgn and vsgn wind arrays"; return CSADV_E_ARG; }
-  if (d.mf == 3 && (!d.corr || !d.partials)) { msg = "MF-PR needs corr and partials"; return CSADV_E_ARG; }
+  if (d.mf == 3 && (!d.corr || !d.partials || !d.gfix)) { msg = "MF-PR needs corr, partials and gfix"; return CSADV_E_ARG; }
   if (d.n_strip != 6 * 4 * 3 * d.N || d.n_corner != 6 * 2 * 3 * 6 || !d.strip_tgt || !d.strip_src || !d.strip_w ||
-      !d.strip_gw || !d.corner_tgt || !d.corner_src || !d.corner_w || !d.corner_gw) {
+      !d.corner_tgt || !d.corner_src || !d.corner_w) {
     msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";
     return CSADV_E_ARG;
   }
@@ -993,7 +996,8 @@ int step_ghosts(csadv_t *h, hipStream_t st) {
   for (auto &d : h->desc) maxc = std::max(maxc, d.n_strip + d.n_corner);
   int maxtr = 1;
   for (auto &d : h->desc) maxtr = std::max(maxtr, d.ntr);
-  const dim3 grid((maxc + BLOCK - 1) / BLOCK, ns, maxtr);
+  // recipe blocks of the largest solve, then the block of the correction total
+  const dim3 grid((maxc + BLOCK - 1) / BLOCK + 1, ns, maxtr);
   const bool one = ns == 1;
   if (h->dtype == CSADV_F64) {
     if (one) hipLaunchKernelGGL(k_ghost1<double>, grid, dim3(BLOCK), 0, st, h->host[0]);

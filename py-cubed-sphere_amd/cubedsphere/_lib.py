@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcsadv.so")
 
 CSADV_F64, CSADV_F32 = 0, 1
-ABI_VERSION = 4          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
+ABI_VERSION = 5          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
 _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of order", -4: "size out of range"}
 
 # exported symbols, in declaration order of include/csadv.h
@@ -35,12 +35,12 @@ class SolveDesc(ctypes.Structure):
         ("nbasis", ctypes.c_int32), ("unsteady", ctypes.c_int32), ("max_steps", ctypes.c_int32),
         ("dt", ctypes.c_double), ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("a2", ctypes.c_double),
         ("q0", _p), ("q1", _p), ("uavg", _p), ("vavg", _p), ("usgn", _p), ("vsgn", _p),
-        ("gc", _p), ("gu", _p), ("gv", _p), ("rgc", _p), ("gc64", _p),
+        ("gc", _p), ("gu", _p), ("gv", _p), ("rgc", _p), ("gfix", _p), ("gc64", _p),
         ("lagw", _p), ("lagk", _p), ("orient", _p), ("wbu", _p), ("wbv", _p),
         ("conv_pc", _p), ("conv_pu", _p), ("conv_pv", _p), ("wpc", _p),
         ("n_strip", ctypes.c_int32), ("n_corner", ctypes.c_int32),
-        ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p), ("strip_gw", _p),
-        ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p), ("corner_gw", _p),
+        ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p),
+        ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p),
         ("coef", _p), ("corr", _p), ("partials", _p), ("ctl", _p),
         ("tile", ctypes.c_int32), ("stop", ctypes.c_int32), ("work", _p),
     ]

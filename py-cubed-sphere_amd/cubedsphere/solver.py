@@ -83,14 +83,37 @@ class GridTables:
         self.n_strip, self.n_corner = len(rec["strip_tgt"]), len(rec["corner_tgt"])
         self.strip_tgt, self.strip_src = up(rec["strip_tgt"], torch.int32), up(rec["strip_src"], torch.int32)
         self.strip_w = up(rec["strip_w"])
-        self.strip_gw = up(np.sum(rec["strip_w"] * gflat[rec["strip_gsrc"]], axis=1))
         self.corner_tgt, self.corner_src = up(rec["corner_tgt"], torch.int32), up(rec["corner_src"], torch.int32)
         self.corner_w = up(rec["corner_w"])
-        self.corner_gw = up(np.sum(rec["corner_w"] * gflat[rec["corner_gsrc"]], axis=1))
+        self.gfix = up(correction_metric(rec, gflat, M * ld).reshape(M, ld), td)
         pc, pu, pv = grid.pc, grid.pu, grid.pv
         self.conv_pc = up(_pad_last(_stack([pc.ex_lon, pc.ex_lat, pc.ey_lon, pc.ey_lat]), ld))
         self.conv_pu = up(_pad_last(_stack([pu.ex_lon, pu.ex_lat, pu.ey_lon, pu.ey_lat, pu.det]), ldu))
         self.conv_pv = up(_pad_last(_stack([pv.ex_lon, pv.ex_lat, pv.ey_lon, pv.ey_lat, pv.det]), ld))
+
+
+def correction_metric(rec, gflat, pstride):
+    """The metric the mass-fixer correction total C multiplies at every cell a step reads:
+    g inside the panel and, at a ghost cell, sum_l w_l g(src_l) over its recipe, so that a
+    ghost value interpolated from the stored values S_l takes the correction as
+    sum_l w_l (S_l + C g_l) = sum_l w_l S_l + C gfix.  By the cube's symmetry the sum is the
+    same at a ghost position of every panel (checked here); panel 0's values are kept."""
+    gfix = np.array(gflat, dtype=np.float64, copy=True)
+    for kind in ("strip", "corner"):
+        gw = np.sum(rec[kind + "_w"] * gflat[rec[kind + "_gsrc"]], axis=1)
+        tgt = rec[kind + "_tgt"].astype(np.int64)
+        loc = tgt % pstride
+        lo = np.full(pstride, np.inf)
+        hi = np.full(pstride, -np.inf)
+        np.minimum.at(lo, loc, gw)
+        np.maximum.at(hi, loc, gw)
+        seen = np.isfinite(lo)
+        spread = np.max(hi[seen] - lo[seen])
+        if spread > 1e-12 * np.max(np.abs(gflat)):
+            raise RuntimeError("ghost metric weights differ between panels by %.3e" % spread)
+        first = tgt < pstride
+        gfix[tgt[first]] = gw[first]
+    return gfix
 
 
 _TABLES = {}
@@ -232,15 +255,14 @@ class Advection:
         d.q0, d.q1 = _ptr(self.q[0]), _ptr(self.q[1])
         d.uavg, d.vavg, d.usgn, d.vsgn = _ptr(self.uavg), _ptr(self.vavg), _ptr(self.usgn), _ptr(self.vsgn)
         d.gc, d.gu, d.gv, d.rgc, d.gc64 = _ptr(tab.gc), _ptr(tab.gu), _ptr(tab.gv), _ptr(tab.rgc), _ptr(tab.gc64)
+        d.gfix = _ptr(tab.gfix)
         d.lagw, d.lagk, d.orient = _ptr(tab.lagw), _ptr(tab.lagk), _ptr(tab.orient)
         d.wbu, d.wbv = _ptr(self.wbu), _ptr(self.wbv)
         d.conv_pc, d.conv_pu, d.conv_pv = _ptr(tab.conv_pc), _ptr(tab.conv_pu), _ptr(tab.conv_pv)
         d.wpc = _ptr(self.wpc)      # null once the winds are prepared (scratch released)
         d.n_strip, d.n_corner = tab.n_strip, tab.n_corner
-        d.strip_tgt, d.strip_src, d.strip_w, d.strip_gw = (_ptr(tab.strip_tgt), _ptr(tab.strip_src), _ptr(tab.strip_w),
-                                                           _ptr(tab.strip_gw))
-        d.corner_tgt, d.corner_src, d.corner_w, d.corner_gw = (_ptr(tab.corner_tgt), _ptr(tab.corner_src),
-                                                               _ptr(tab.corner_w), _ptr(tab.corner_gw))
+        d.strip_tgt, d.strip_src, d.strip_w = _ptr(tab.strip_tgt), _ptr(tab.strip_src), _ptr(tab.strip_w)
+        d.corner_tgt, d.corner_src, d.corner_w = _ptr(tab.corner_tgt), _ptr(tab.corner_src), _ptr(tab.corner_w)
         d.coef, d.corr, d.partials, d.ctl = _ptr(self.coef), _ptr(self.corr), _ptr(self.partials), _ptr(self.ctl)
         d.tile = self.tile
         d.stop = self.stop

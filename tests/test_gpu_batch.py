@@ -105,7 +105,7 @@ def test_device_built_preparation_matches_host(N):
     from cubedsphere.cases import workload
     from cubedsphere.solver import grid_tables
     h, d = grid_tables(N, "f64", "cuda:0"), grid_tables(N, "f64", "cuda:0", device_grid=True)
-    for name in ("gc", "gu", "gv", "rgc", "conv_pc", "conv_pu", "conv_pv", "strip_w", "corner_gw"):
+    for name in ("gc", "gu", "gv", "rgc", "conv_pc", "conv_pu", "conv_pv", "strip_w", "gfix"):
         a, b = getattr(h, name), getattr(d, name)
         assert torch.allclose(a, b, rtol=0, atol=4e-15 * float(a.abs().max())), name
     assert torch.equal(h.strip_src, d.strip_src) and torch.equal(h.corner_src, d.corner_src)
