@@ -285,6 +285,7 @@ struct StepIn {     // inputs of iteration y at the lane's column: row y and y-e
   T q, u, us, gu, gc, rg;    // row y: Q, x-edge wind, upwind-side wind, metric, 1/g
   T gf;                      // row y: metric the correction total multiplies (gfix)
   T raw;                     // row y: the stored value before the correction total is added
+  T qa;                      // first rows only: Q of row y in the other state buffer
   T v, vs, gvp;              // y-edge y-3: wind, upwind-side wind; metric of y-edge y-2
 };
 
@@ -298,7 +299,7 @@ struct March {
   static constexpr int PF = 2;
   static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
-  const T *Q, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
+  const T *Q, *Qa, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
   T *Qo;
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
   bool out_lane, need_col, mfpr, corrects;
@@ -312,6 +313,9 @@ struct March {
   T pa_A, pa_B, pa_q6, pb_A, pb_B, pb_q6;   // row y-4 as the left (upwind) cell of edge y-3: Q, Qx
   double part;
 
+  // SPEC: also read Q from the other state buffer (Qa), for the rows requested before the step
+  // counter has arrived
+  template <bool SPEC = false>
   __device__ __forceinline__ void load(int y, StepIn<T> &r) const {
     const int yc = min(y, M - 1);
     const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
@@ -320,16 +324,25 @@ struct March {
     r.gu = ld_at(gu, ou);
     r.gc = ld_at(gc, o);
     r.rg = ld_at(rgc, o);
-    if (corrects) r.gf = ld_at(gfix, o);
+    if (mfpr) r.gf = ld_at(gfix, o);
+    const bool reach = need_col && y >= 1 && y < iend + 3;
     if constexpr (std::is_same<T, float>::value) {
       // FP32: Q of every lane is read (clamped column; rows y0..ylast lie in 1..M-1) and zeroed
       // outside the stencils' reach by a select, with no branch around the load (measured
       // faster for FP32, slower for FP64, whose registers it raises)
       const T qv = ld_at(Q, (uint32_t)(y * ld + xc));
-      r.q = (need_col && y >= 1 && y < iend + 3) ? qv : T(0);
+      r.q = reach ? qv : T(0);
+      if constexpr (SPEC) {
+        const T qa = ld_at(Qa, (uint32_t)(y * ld + xc));
+        r.qa = reach ? qa : T(0);
+      }
     } else {
       r.q = T(0);    // rows outside the stencils' reach
-      if (need_col && y >= 1 && y < iend + 3) r.q = ld_at(Q, (uint32_t)(y * ld + x));
+      r.qa = T(0);
+      if (reach) {
+        r.q = ld_at(Q, (uint32_t)(y * ld + x));
+        if constexpr (SPEC) r.qa = ld_at(Qa, (uint32_t)(y * ld + x));
+      }
     }
     const int ec = min(max(y - 3, 0), M - 1);
     r.v = ld_at(V, (uint32_t)(ec * ld + xc));
@@ -425,18 +438,29 @@ struct March {
     if (y + PF <= ylast) load(y + PF, in);
   }
 
-  __device__ __forceinline__ void run() {
+  // The first PF rows' inputs, requested before the step counter is known: Q from both state
+  // buffers (Q = q0, Qa = q1), the rest from the step-invariant tables and winds.
+  __device__ __forceinline__ void start(StepIn<T> *b) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = sw[k] = T(0);
     gvm1 = gv0 = gvp1 = T(0);     // complete from y = tj-1 on; the column stage starts at tj+2
     g1prev = g2prev = vprev = T(0);
     pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
     part = 0.0;
-    StepIn<T> b[PF];
     const int y0 = tj - 3;            // ylast - y0 = ny + 6 >= PF: every initial row exists
 #pragma unroll
-    for (int k = 0; k < PF; ++k) load(y0 + k, b[k]);
-    int y = y0;
+    for (int k = 0; k < PF; ++k) load<true>(y0 + k, b[k]);
+  }
+
+  // The step's input buffer is known: keep the first rows' Q from it and read the rest there.
+  __device__ __forceinline__ void resolve(StepIn<T> *b, bool odd) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) b[k].q = odd ? b[k].qa : b[k].q;
+    Q = odd ? Qa : Q;
+  }
+
+  __device__ __forceinline__ void run(StepIn<T> *b) {
+    int y = tj - 3;
     for (; y + 5 <= ylast; y += 6) {
       body<0>(y, b[0]); body<1>(y + 1, b[1 % PF]); body<2>(y + 2, b[2 % PF]);
       body<3>(y + 3, b[3 % PF]); body<4>(y + 4, b[4 % PF]); body<5>(y + 5, b[5 % PF]);
@@ -458,14 +482,8 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int ti = NG + wk.z * S.strip_w, tj = NG + wk.w * S.seg_l;
   const int iend = N + NG;
   const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
-  const int step = D.ctl[2];
-  if (D.stop > 0 && step >= D.stop) return;   // solve finished
   const int item = (p * S.segs + wk.w) * S.strips + wk.z;
   const bool mfpr = D.mf == 3;
-  if (item == 0 && tr == 0 && lane == 0) {   // hand the step over to the next kernels of the stream
-    D.ctl[1] = step + 1;
-    if (mfpr) D.ctl[3] = 1;
-  }
   March<T, RECON, RK2, SPLIT> m;
   m.ld = D.ld;
   m.ldu = D.ldu;
@@ -491,16 +509,29 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.gv = reinterpret_cast<const T *>(D.gv);
   m.rgc = reinterpret_cast<const T *>(D.rgc);
   m.gfix = reinterpret_cast<const T *>(D.gfix);
-  const T *Qin = reinterpret_cast<const T *>((step & 1) ? D.q1 : D.q0);
-  T *Qout = reinterpret_cast<T *>((step & 1) ? D.q0 : D.q1);
-  m.Q = Qin + tr * S.tstride + p * S.pstride;
-  m.Qo = Qout + tr * S.tstride + p * S.pstride;
-  m.cf = D.corr ? D.corr[(step & 1) * D.ntr + tr] : 0.0;
+  const int64_t off = tr * S.tstride + p * S.pstride;
+  m.Q = reinterpret_cast<const T *>(D.q0) + off;
+  m.Qa = reinterpret_cast<const T *>(D.q1) + off;
+  StepIn<T> b[March<T, RECON, RK2, SPLIT>::PF];
+  m.start(b);
+  // read after the first rows are requested: a wait for any scalar load waits for all of them
+  // (they return out of order), so reading these first would hold back the rows' loads
+  const int step = D.ctl[2];
+  const double c0 = mfpr ? D.corr[tr] : 0.0, c1 = mfpr ? D.corr[D.ntr + tr] : 0.0;
+  if (D.stop > 0 && step >= D.stop) return;   // solve finished
+  const bool odd = step & 1;                  // the state of step n lives in q[n & 1]
+  m.resolve(b, odd);
+  m.Qo = reinterpret_cast<T *>(odd ? D.q0 : D.q1) + off;
+  m.cf = odd ? c1 : c0;
   m.corrects = m.cf != 0.0;
-  m.run();
+  m.run(b);
   if (mfpr) {
     const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
     if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
+  }
+  if (item == 0 && tr == 0 && lane == 0) {   // hand the step over to the next kernels of the stream
+    D.ctl[1] = step + 1;
+    if (mfpr) D.ctl[3] = 1;
   }
 }
 
@@ -511,12 +542,26 @@ __global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves,
   step_item<T, RECON, RK2, SPLIT>(solves[wk.x], wk);
 }
 
-// A handle of one solve: the descriptor travels in the kernel arguments, so the work item and
-// the step counter are requested at once instead of after a descriptor load.
+// Work item b of a single-solve handle, computed instead of read: the list the host builds
+// (csadv_update) gives workgroup b, which runs on XCD b % kXcds, item b / kXcds of that XCD's
+// band; band x holds the row segments sg with sg * kXcds / segs == x, each segment listing
+// its (panel, strip, tracer) items with the tracer fastest.  Past the band's end: padding.
+__device__ __forceinline__ bool single_item(const Solve &S, int b, int4 &wk) {
+  const int x = b % kXcds, i = b / kXcds;
+  const int lo = (x * S.segs + kXcds - 1) / kXcds, hi = ((x + 1) * S.segs + kXcds - 1) / kXcds;
+  const int ntr = S.d.ntr, row = S.strips * ntr, per = 6 * row;
+  if (i >= (hi - lo) * per) return false;
+  const int r = i % per, r2 = r % row;
+  wk = make_int4(0, r / row + 8 * (r2 % ntr), r2 / ntr, lo + i / per);
+  return true;
+}
+
+// A handle of one solve: the descriptor travels in the kernel arguments and the work item is
+// computed, so the first loads of a wavefront depend on nothing but the arguments.
 template <typename T, int RECON, bool RK2, int SPLIT>
-__global__ __launch_bounds__(WAVE) void k_step1(const Solve one, const int4 *work) {
-  const int4 wk = work[blockIdx.x];
-  if (wk.x < 0) return;
+__global__ __launch_bounds__(WAVE) void k_step1(const Solve one) {
+  int4 wk;
+  if (!single_item(one, blockIdx.x, wk)) return;
   step_item<T, RECON, RK2, SPLIT>(one, wk);
 }
 
@@ -879,7 +924,6 @@ void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs) {
   segs = (N + seg_l - 1) / seg_l;
 }
 constexpr int kRedBlocks = 256;
-constexpr int kXcds = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin over them
 
 int fail(csadv_t *h, int code, const std::string &msg) {
   if (h) h->err = msg;
@@ -957,10 +1001,10 @@ template <typename T, int R, int SP>
 hipError_t launch_step_s(csadv_t *h, hipStream_t st) {
   const bool one = h->host.size() == 1;
   if (h->rk2) {
-    if (one) hipLaunchKernelGGL((k_step1<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0], h->work);
+    if (one) hipLaunchKernelGGL((k_step1<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
     else hipLaunchKernelGGL((k_step<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
   } else {
-    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0], h->work);
+    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
     else hipLaunchKernelGGL((k_step<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
   }
   return hipGetLastError();

@@ -24,6 +24,7 @@ struct Solve {
 };
 
 constexpr int WAVE = 64;
+constexpr int kXcds = 8;             // MI355X: 8 XCDs; workgroup b is dispatched to XCD b % 8
 constexpr int STRIP_MAX = WAVE - 6;  // interior columns of one wavefront (3-cell halo each side)
 
 // A set of FP64 or dtype fields that receive a duo-grid ghost fill.
