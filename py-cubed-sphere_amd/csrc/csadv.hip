@@ -396,7 +396,7 @@ struct March {
           const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
           const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
           const Parab<T> c = parabola<T, RECON>(qy_m2, qy_m1, qy, qy_p1, qy_p2, gur, from_right(gur), gcm, mt0);
-          const T f2 = upwind<T>(urp, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, ur * dtdx, gur, mt0);
+          const T f2 = upwind_x<T>(urp, c, ur * dtdx, gur, from_right(gur), mt0);
           const T dF2 = f2 - from_right(f2);
           const T dG2 = g2prev - g2;
           // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
@@ -424,7 +424,7 @@ struct March {
     const T q_m1 = from_left(q), q_p1 = from_right(q);
     const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
     const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
-    const T f1 = upwind<T>(up, from_left(c.rA), from_left(c.rB), from_left(c.q6), c, u * dtdx, gu0, mt0);
+    const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);
     const T dF1 = f1 - from_right(f1);
     T c1 = T(0), c2 = T(0);
     if (split != 1) {

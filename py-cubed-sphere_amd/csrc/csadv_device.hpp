@@ -111,10 +111,12 @@ __device__ __forceinline__ void recon_edges(T m2, T m1, T c0, T p1, T p2, T &qL,
 
 // The parabola of one cell as the upwind flux uses it (src/flux.py:28-44): edge values
 // from the 5-cell window, with the MT-0 metric applied (gl / gr at the cell's left and right
-// edges, gcell at its centre), dq = q_R - q_L and q6 = 6q - 3(q_R + q_L).  An edge takes it
-// from the cell on its left when the wind there is non-negative (rA = q_R, rB = q6 - dq)
-// and from the cell on its right otherwise (lA = q_L, lB = q6 + dq).  Each cell's parabola
-// is formed once and an edge only selects three values from one side.
+// edges, gcell at its centre).  With dq = q_R - q_L and q6 = 6q - 3(q_R + q_L), it is kept in
+// the scaled form the flux below evaluates fastest: with a = 3q - (q_R + q_L),
+//   rB = a - q_R = (q6 - dq) / 2,   lB = a - q_L = (q6 + dq) / 2,   q6' = a - q = q6 / 3.
+// An edge takes it from the cell on its left when the wind there is non-negative
+// (A = q_R, B = rB) and from the cell on its right otherwise (A = q_L, B = -lB).  Each cell's
+// parabola is formed once and an edge only selects three values from one side.
 template <typename T>
 struct Parab {
   T rA, rB, lA, lB, q6;
@@ -126,9 +128,8 @@ __device__ __forceinline__ Parab<T> parabola(T m2, T m1, T c0, T p1, T p2, T gl,
   recon_edges<T, RECON>(m2, m1, c0, p1, p2, qL, qR);
   T q = c0;
   if (mt0) { qL = qL * gl; qR = qR * gr; q = q * gcell; }
-  const T dq = qR - qL;
-  const T q6 = T(6) * q - T(3) * (qR + qL);
-  return {qR, q6 - dq, qL, q6 + dq, q6};
+  const T a = T(3) * q - (qR + qL);
+  return {qR, a - qR, qL, a - qL, a - q};
 }
 
 // Upwind PPM flux through one edge (src/flux.py:48-70) from the parabolas of the cells on its
@@ -136,13 +137,12 @@ __device__ __forceinline__ Parab<T> parabola(T m2, T m1, T c0, T p1, T p2, T gl,
 // f = q_L - (c/2)(q6 + dq) - q6 c^2/3 from the right cell otherwise; F = f u_avg, and
 // MT-PL07 multiplies by the edge metric ge.  Returned already times dt/dx, as F dt/dx =
 // f c (c = u_avg dt/dx, the Courant number), so the flux difference of a cell is the
-// difference of two returned values; with h = c/2, f = A + h (B - (4/3) q6 h), which takes
-// five operations where the textbook order takes seven.
+// difference of two returned values.  In the scaled form of the parabola,
+// f = A + c (B - q6' c): three operations with the final product by c.
 template <typename T>
 __device__ __forceinline__ T upwind(bool up, T LrA, T LrB, T Lq6, const Parab<T> &R, T cc, T ge, bool mt0) {
   const T A = up ? LrA : R.lA, B = up ? LrB : -R.lB, q6 = up ? Lq6 : R.q6;
-  const T h = cc * T(0.5);
-  T f = (A + h * (B - T(4.0 / 3.0) * (q6 * h))) * cc;
+  T f = (A + cc * (B - q6 * cc)) * cc;
   if (!mt0) f = f * ge;
   return f;
 }
@@ -167,6 +167,23 @@ __device__ __forceinline__ double dpp_move(double v) {
 }
 template <typename T> __device__ __forceinline__ T from_left(T v) { return dpp_move<kDppWaveShr1>(v); }
 template <typename T> __device__ __forceinline__ T from_right(T v) { return dpp_move<kDppWaveShl1>(v); }
+
+// Flux through the left edge of each lane's cell along the wavefront (x), lane = cell: every
+// lane forms the flux its cell sends through its right edge when it is the upwind cell there
+// (c_r = c of the right neighbour) and through its left edge when it is the upwind cell there;
+// the left edge then takes the left neighbour's right-edge value when the wind there is
+// non-negative and the lane's own value otherwise.  One shift of the result replaces shifting
+// the left cell's three parabola values (the same arithmetic as upwind(), so the same bits).
+// gl, gr: edge metrics at the left and right edges (MT-PL07 only).
+template <typename T>
+__device__ __forceinline__ T upwind_x(bool up, const Parab<T> &c, T cc, T gl, T gr, bool mt0) {
+  const T ccr = from_right(cc);
+  T fr = (c.rA + ccr * (c.rB - c.q6 * ccr)) * ccr;
+  T fl = (c.lA + cc * (-c.lB - c.q6 * cc)) * cc;
+  if (!mt0) { fr = fr * gr; fl = fl * gl; }
+  const T from_l = from_left(fr);
+  return up ? from_l : fl;
+}
 
 // Element idx of an array whose base is wave-uniform: the byte offset stays 32-bit so that
 // the load takes the scalar-base + vector-offset form (callers keep idx * sizeof(T) < 4 GiB).
