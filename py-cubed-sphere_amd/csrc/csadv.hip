@@ -282,11 +282,11 @@ __global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
 // src/advection_timestep.py:43)
 template <typename T>
 struct StepIn {     // inputs of iteration y at the lane's column: row y and y-edge y-3
-  T q, u, us, gu, gc, rg;    // row y: Q, x-edge wind, upwind-side wind, metric, 1/g
+  T q;                       // row y: Q
   T gf;                      // row y: metric the correction total multiplies (gfix)
   T raw;                     // row y: the stored value before the correction total is added
   T qa;                      // first rows only: Q of row y in the other state buffer
-  T v, vs, gvp;              // y-edge y-3: wind, upwind-side wind; metric of y-edge y-2
+  T v, vs;                   // y-edge y-3: wind, upwind-side wind
 };
 
 // SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07); the metric treatment follows from it, since
@@ -305,25 +305,29 @@ struct March {
   bool out_lane, need_col, mfpr, corrects;
   T dtdx, dtdy;
   double cf;
-  // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6)
-  T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6];
+  // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6).
+  // The step-invariant row inputs (metric, 1/g, winds) are loaded straight into the slot their
+  // row will hold: with PF = 2 the loads of row y+2 are issued at the end of iteration y, when
+  // that slot's previous row, y-4, has had its last use.  gvr holds gv at y-edges by the same
+  // phase (edge y-2 in slot K % 6), whose previous edge, y-6, was last read at iteration y-2.
+  T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6], gvr[6];
   T sw[6];                   // FP32: stored values S of the rows (the update is S + flux)
-  T gvm1, gv0, gvp1;         // gv at y-edges e-1, e, e+1 (e = y-3)
   T g1prev, g2prev, vprev;
   T pa_A, pa_B, pa_q6, pb_A, pb_B, pb_q6;   // row y-4 as the left (upwind) cell of edge y-3: Q, Qx
   double part;
 
-  // SPEC: also read Q from the other state buffer (Qa), for the rows requested before the step
-  // counter has arrived
-  template <bool SPEC = false>
-  __device__ __forceinline__ void load(int y, StepIn<T> &r) const {
+  // Inputs of row y, whose ring slot is SLOT.  SPEC: also read Q from the other state buffer
+  // (Qa), for the rows requested before the step counter has arrived.
+  template <bool SPEC, int SLOT>
+  __device__ __forceinline__ void load(int y, StepIn<T> &r) {
+    static_assert(PF == 2, "the ring slots are reused two rows ahead");
     const int yc = min(y, M - 1);
     const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
-    r.u = ld_at(U, ou);
-    r.us = RK2 ? ld_at(US, ou) : r.u;
-    r.gu = ld_at(gu, ou);
-    r.gc = ld_at(gc, o);
-    r.rg = ld_at(rgc, o);
+    uw[SLOT] = ld_at(U, ou);
+    if constexpr (RK2) usw[SLOT] = ld_at(US, ou);
+    guw[SLOT] = ld_at(gu, ou);
+    gcw[SLOT] = ld_at(gc, o);
+    rgw[SLOT] = ld_at(rgc, o);
     if (mfpr) r.gf = ld_at(gfix, o);
     const bool reach = need_col && y >= 1 && y < iend + 3;
     if constexpr (std::is_same<T, float>::value) {
@@ -347,8 +351,11 @@ struct March {
     const int ec = min(max(y - 3, 0), M - 1);
     r.v = ld_at(V, (uint32_t)(ec * ld + xc));
     r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
-    r.gvp = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
+    gvr[SLOT] = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
   }
+
+  // the upwind-side x-wind of the row in a slot (RK1: the wind itself)
+  __device__ __forceinline__ T wind_sign(int slot) const { return RK2 ? usw[slot] : uw[slot]; }
 
   // every cell the stencils read carries the mass-fixer correction total C: Q = S + C gfix,
   // gfix being g inside the panel and, at a ghost cell, the weighted metric of the cells its
@@ -365,9 +372,7 @@ struct March {
     constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
                   s5 = (K + 5) % 6;
     correct(in);
-    gvm1 = gv0;
-    gv0 = gvp1;
-    gvp1 = in.gvp;
+    const T gvm1 = gvr[s4], gv0 = gvr[s5], gvp1 = gvr[s0];   // gv at y-edges y-4, y-3, y-2
     // ---- column stage: parabolas of row y-3 (from y = tj+2 on), y-edge e = y-3 (from tj+3 on),
     // output row r = y-4 (from tj+4 on)
     if (y >= tj + 2) {
@@ -391,7 +396,7 @@ struct March {
           const T rg = rgw[s2];
           const T qy = split_update<T>(split, qw[s2], dG1, gcm, rg, d1, d2);
           const T ur = uw[s2];
-          const bool urp = usw[s2] >= T(0);
+          const bool urp = wind_sign(s2) >= T(0);
           const T gur = guw[s2];
           const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
           const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
@@ -418,8 +423,8 @@ struct March {
       pb_A = b.rA; pb_B = b.rB; pb_q6 = b.q6;
     }
     // ---- row stage: row y
-    const T q = in.q, u = in.u, gu0 = in.gu, gc0 = in.gc;
-    const bool up = in.us >= T(0);
+    const T q = in.q, u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
+    const bool up = wind_sign(s0) >= T(0);
     const T gup1 = from_right(gu0);
     const T q_m1 = from_left(q), q_p1 = from_right(q);
     const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
@@ -431,25 +436,24 @@ struct March {
       c1 = gup1 * (from_right(u) * dtdx);
       c2 = gu0 * (u * dtdx);
     }
-    const T qx = split_update<T>(split, q, dF1, gc0, in.rg, c1, c2);
-    // row y replaces row y-6 in every ring
-    qw[s0] = q; xw[s0] = qx; gcw[s0] = gc0; rgw[s0] = in.rg; uw[s0] = u; usw[s0] = in.us; guw[s0] = gu0;
+    const T qx = split_update<T>(split, q, dF1, gc0, rgw[s0], c1, c2);
+    // row y replaces row y-6 in the rings of computed values (the loaded ones are there already)
+    qw[s0] = q; xw[s0] = qx;
     if constexpr (std::is_same<T, float>::value) sw[s0] = in.raw;
-    if (y + PF <= ylast) load(y + PF, in);
+    if (y + PF <= ylast) load<false, (K + PF) % 6>(y + PF, in);
   }
 
   // The first PF rows' inputs, requested before the step counter is known: Q from both state
   // buffers (Q = q0, Qa = q1), the rest from the step-invariant tables and winds.
   __device__ __forceinline__ void start(StepIn<T> *b) {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = sw[k] = T(0);
-    gvm1 = gv0 = gvp1 = T(0);     // complete from y = tj-1 on; the column stage starts at tj+2
-    g1prev = g2prev = vprev = T(0);
+    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = gvr[k] = sw[k] = T(0);
+    g1prev = g2prev = vprev = T(0);   // gv is complete from y = tj-1 on; the column stage starts at tj+2
     pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
     part = 0.0;
-    const int y0 = tj - 3;            // ylast - y0 = ny + 6 >= PF: every initial row exists
-#pragma unroll
-    for (int k = 0; k < PF; ++k) load<true>(y0 + k, b[k]);
+    // y0 = tj - 3 has phase 0; ylast - y0 = ny + 6 >= PF: every initial row exists
+    load<true, 0>(tj - 3, b[0]);
+    load<true, 1>(tj - 2, b[1]);
   }
 
   // The step's input buffer is known: keep the first rows' Q from it and read the rest there.
