@@ -331,13 +331,13 @@ struct March {
     if (mfpr) r.gf = ld_at(gfix, o);
     const bool reach = need_col && y >= 1 && y < iend + 3;
     if constexpr (std::is_same<T, float>::value) {
-      // FP32: Q of every lane is read (clamped column; rows y0..ylast lie in 1..M-1) and zeroed
-      // outside the stencils' reach by a select, with no branch around the load (measured
-      // faster for FP32, slower for FP64, whose registers it raises)
-      const T qv = ld_at(Q, (uint32_t)(y * ld + xc));
+      // FP32: Q of every lane is read (clamped row and column: rows past ylast are requested
+      // too) and zeroed outside the stencils' reach by a select, with no branch around the load
+      // (measured faster for FP32, slower for FP64, whose registers it raises)
+      const T qv = ld_at(Q, o);
       r.q = reach ? qv : T(0);
       if constexpr (SPEC) {
-        const T qa = ld_at(Qa, (uint32_t)(y * ld + xc));
+        const T qa = ld_at(Qa, o);
         r.qa = reach ? qa : T(0);
       }
     } else {
@@ -367,25 +367,29 @@ struct March {
     if (corrects) r.q = r.q + T(cf) * r.gf;
   }
 
-  template <int K>
+  // Iteration y in phase K.  STAGE: how much of the column stage has begun, fixed by the row's
+  // offset from the item's first row y0 = tj - 3: parabolas of row y-3 from y = tj+2 on
+  // (STAGE >= 1), fluxes through y-edge y-3 from tj+3 on (>= 2), the update of row y-4 from
+  // tj+4 on (3).  The first seven rows are peeled with their stage; every later row is 3, so
+  // the loop carries no per-row test.
+  template <int K, int STAGE>
   __device__ __forceinline__ void body(const int y, StepIn<T> &in) {
     constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
                   s5 = (K + 5) % 6;
     correct(in);
     const T gvm1 = gvr[s4], gv0 = gvr[s5], gvp1 = gvr[s0];   // gv at y-edges y-4, y-3, y-2
-    // ---- column stage: parabolas of row y-3 (from y = tj+2 on), y-edge e = y-3 (from tj+3 on),
-    // output row r = y-4 (from tj+4 on)
-    if (y >= tj + 2) {
+    // ---- column stage: parabolas of row y-3, y-edge e = y-3, output row r = y-4
+    if constexpr (STAGE >= 1) {
       const T gce = gcw[s3];
       const Parab<T> a = parabola<T, RECON>(qw[s1], qw[s2], qw[s3], qw[s4], qw[s5], gv0, gvp1, gce, mt0);
       const Parab<T> b = parabola<T, RECON>(xw[s1], xw[s2], xw[s3], xw[s4], xw[s5], gv0, gvp1, gce, mt0);
-      if (y >= tj + 3) {
+      if constexpr (STAGE >= 2) {
         const T v = in.v;
         const bool vup = in.vs >= T(0);
         const T cy = v * dtdy;
         const T g1 = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, gv0, mt0);     // times dt/dy
         const T g2 = upwind<T>(vup, pb_A, pb_B, pb_q6, b, cy, gv0, mt0);
-        if (y >= tj + 4) {
+        if constexpr (STAGE >= 3) {
           const T gcm = gcw[s2];               // row y-4
           const T dG1 = g1prev - g1;
           T d1 = T(0), d2 = T(0);
@@ -406,7 +410,7 @@ struct March {
           const T dG2 = g2prev - g2;
           // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
           const T flx = dF2 + dG2;
-          if (out_lane && y <= ylast) {
+          if (out_lane) {
             // the correction total stays out of the stored state: S(n+1) = S(n) + flux
             T out;
             if constexpr (std::is_same<T, float>::value) out = sw[s2] + flx * rg;
@@ -440,7 +444,8 @@ struct March {
     // row y replaces row y-6 in the rings of computed values (the loaded ones are there already)
     qw[s0] = q; xw[s0] = qx;
     if constexpr (std::is_same<T, float>::value) sw[s0] = in.raw;
-    if (y + PF <= ylast) load<false, (K + PF) % 6>(y + PF, in);
+    // rows past ylast are read in bounds (clamped rows, masked Q) and never used
+    load<false, (K + PF) % 6>(y + PF, in);
   }
 
   // The first PF rows' inputs, requested before the step counter is known: Q from both state
@@ -464,16 +469,20 @@ struct March {
   }
 
   __device__ __forceinline__ void run(StepIn<T> *b) {
+    // rows y0 .. y0+6 (ylast - y0 = ny + 6 >= 7: they all exist), the column stage starting
     int y = tj - 3;
-    for (; y + 5 <= ylast; y += 6) {
-      body<0>(y, b[0]); body<1>(y + 1, b[1 % PF]); body<2>(y + 2, b[2 % PF]);
-      body<3>(y + 3, b[3 % PF]); body<4>(y + 4, b[4 % PF]); body<5>(y + 5, b[5 % PF]);
+    body<0, 0>(y, b[0]); body<1, 0>(y + 1, b[1]); body<2, 0>(y + 2, b[0]); body<3, 0>(y + 3, b[1]);
+    body<4, 0>(y + 4, b[0]); body<5, 1>(y + 5, b[1]); body<0, 2>(y + 6, b[0]);
+    // rows y0+7 on: the full step, six rows (phases 1 .. 5, 0) per iteration
+    for (y += 7; y + 5 <= ylast; y += 6) {
+      body<1, 3>(y, b[1]); body<2, 3>(y + 1, b[0]); body<3, 3>(y + 2, b[1]);
+      body<4, 3>(y + 3, b[0]); body<5, 3>(y + 4, b[1]); body<0, 3>(y + 5, b[0]);
     }
-    if (y <= ylast) body<0>(y, b[0]);
-    if (y + 1 <= ylast) body<1>(y + 1, b[1 % PF]);
-    if (y + 2 <= ylast) body<2>(y + 2, b[2 % PF]);
-    if (y + 3 <= ylast) body<3>(y + 3, b[3 % PF]);
-    if (y + 4 <= ylast) body<4>(y + 4, b[4 % PF]);
+    if (y <= ylast) body<1, 3>(y, b[1]);
+    if (y + 1 <= ylast) body<2, 3>(y + 1, b[0]);
+    if (y + 2 <= ylast) body<3, 3>(y + 2, b[1]);
+    if (y + 3 <= ylast) body<4, 3>(y + 3, b[0]);
+    if (y + 4 <= ylast) body<5, 3>(y + 4, b[1]);
   }
 };
 
