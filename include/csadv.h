@@ -64,10 +64,10 @@ typedef struct csadv_solve_desc {
   void *uavg, *vavg;               /* time-averaged contravariant winds (dtype)         */
   void *usgn, *vsgn;               /* contravariant winds fixing the upwind side (dtype)*/
   const void *gc, *gu, *gv, *rgc;  /* metric tables (dtype)                             */
-  const void *gfix;                /* MF-PR: the metric the correction total multiplies
-                                      (dtype): gc inside the panel, sum_l w_l gc(src_l) of
-                                      the ghost recipe at the 3 ghost layers (the same on
-                                      every panel); the per-step ghost fill stores the
+  const void *gfix;                /* fused step: the metric the MF-PR correction total
+                                      multiplies (dtype): gc inside the panel, sum_l w_l
+                                      gc(src_l) of the ghost recipe at the 3 ghost layers (the
+                                      same on every panel); the per-step ghost fill stores the
                                       uncorrected sums and the step adds C gfix on read     */
   const double *gc64;              /* metric at centres, FP64 (mass fixer, mass)        */
   const double *lagw;              /* Lagrange weights [4][M][4], FP64                  */

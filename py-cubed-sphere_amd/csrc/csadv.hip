@@ -302,7 +302,7 @@ struct March {
   const T *Q, *Qa, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
   T *Qo;
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
-  bool out_lane, need_col, mfpr, corrects;
+  bool out_lane, need_col;
   T dtdx, dtdy;
   double cf;
   // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6).
@@ -328,7 +328,7 @@ struct March {
     guw[SLOT] = ld_at(gu, ou);
     gcw[SLOT] = ld_at(gc, o);
     rgw[SLOT] = ld_at(rgc, o);
-    if (mfpr) r.gf = ld_at(gfix, o);
+    r.gf = ld_at(gfix, o);
     const bool reach = need_col && y >= 1 && y < iend + 3;
     if constexpr (std::is_same<T, float>::value) {
       // FP32: Q of every lane is read (clamped row and column: rows past ylast are requested
@@ -362,9 +362,10 @@ struct March {
   // value was interpolated from (in the solve's precision: C gfix is ~1e-6 of q at most, so
   // its FP32 rounding is far below that of the sum, which is rounded once).  Lanes and rows
   // outside the stencils' reach hold 0 + C gfix, which no stored cell reads.
+  // (with no correction total, cf = 0 and q + 0 gfix = q: no branch on the mass fixer)
   __device__ __forceinline__ void correct(StepIn<T> &r) const {
     r.raw = r.q;
-    if (corrects) r.q = r.q + T(cf) * r.gf;
+    r.q = r.q + T(cf) * r.gf;
   }
 
   // Iteration y in phase K.  STAGE: how much of the column stage has begun, fixed by the row's
@@ -416,7 +417,7 @@ struct March {
             if constexpr (std::is_same<T, float>::value) out = sw[s2] + flx * rg;
             else out = (qw[s2] + flx * rg) - T(cf) * gcm;
             st_at(Qo, (uint32_t)((y - 4) * ld + x), out);
-            if (mfpr) part -= (double)flx;
+            part -= (double)flx;               // used with MF-PR only
           }
         }
         g1prev = g1;
@@ -509,7 +510,6 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.ylast = tj + ny + 3;
   m.out_lane = lane >= 3 && lane < 3 + nx;
   m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
-  m.mfpr = mfpr;
   m.dtdx = T(D.dt / D.dx);
   m.dtdy = T(D.dt / D.dy);
   m.U = reinterpret_cast<const T *>(D.uavg) + p * S.pstride_u;
@@ -536,7 +536,6 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.resolve(b, odd);
   m.Qo = reinterpret_cast<T *>(odd ? D.q0 : D.q1) + off;
   m.cf = odd ? c1 : c0;
-  m.corrects = m.cf != 0.0;
   m.run(b);
   if (mfpr) {
     const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
@@ -980,7 +979,8 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     return CSADV_E_ARG;
   }
   if (d.dp == 2 && (!d.usgn || !d.vsgn)) { msg = "dp=2 needs the usgn and vsgn wind arrays"; return CSADV_E_ARG; }
-  if (d.mf == 3 && (!d.corr || !d.partials || !d.gfix)) { msg = "MF-PR needs corr, partials and gfix"; return CSADV_E_ARG; }
+  if (d.mf == 3 && (!d.corr || !d.partials)) { msg = "MF-PR needs corr and partials"; return CSADV_E_ARG; }
+  if (d.et == 3 && d.mf != 2 && !d.gfix) { msg = "the fused step (ET-DG) needs gfix"; return CSADV_E_ARG; }
   if (d.n_strip != 6 * 4 * 3 * d.N || d.n_corner != 6 * 2 * 3 * 6 || !d.strip_tgt || !d.strip_src || !d.strip_w ||
       !d.corner_tgt || !d.corner_src || !d.corner_w) {
     msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";
