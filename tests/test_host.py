@@ -41,6 +41,36 @@ def test_grid_tables_match_oracle(N):
     assert abs(g.a2 - np.sum(o.metric_tensor_pc[I, I, :] ** 2)) < 1e-12 * g.a2
 
 
+@pytest.mark.parametrize("N", [16, 33])
+def test_correction_metric_is_the_ghost_fill_of_the_metric(N):
+    """gfix, the metric the step multiplies the mass-fixer total by, is g inside every panel and,
+    at each of the 3 ghost layers a step reads (corners included), the oracle's duo-grid ghost
+    fill of the metric field itself (src/interpolation.py:154-315) on every one of the six
+    panels: a ghost value interpolated from S_l + C g_l is then the stored sum plus C gfix."""
+    import torch
+    from oracle import interp as ointerp
+    from cubedsphere.solver import GridTables
+    o = ogrid.CubedSphere(N)
+    tab = GridTables(Grid(N), "f64", torch.device("cpu"))
+    M = N + 8
+    gfix = tab.gfix.numpy()[:, :M].T                       # reference layout [i, j]
+    gc = tab.gc.numpy()[:, :M].T
+    I = slice(o.i0, o.iend)
+    assert np.array_equal(gfix[I, I], gc[I, I])
+    G = np.zeros((M, M, 6))
+    G[I, I, :] = o.metric_tensor_pc[I, I, :]
+    ointerp.ghost_cell_pc_lagrange_interpolation(G, o, olagrange.lagrange_poly_ghostcell_pc(o))
+    reach = np.arange(1, o.iend + 3)                       # columns / rows a step reads
+    ring = np.zeros((M, M), bool)
+    ring[np.ix_(reach, reach)] = True
+    ring[I, I] = False
+    assert ring.sum() == 12 * N + 36
+    for p in range(6):
+        assert np.abs(G[:, :, p][ring] - gfix[ring]).max() <= 1e-14 * np.abs(gc).max(), p
+    # a plausible mistake, the cell's own metric at the ghost positions, is far off
+    assert np.abs(gc[ring] - gfix[ring]).max() > 1e-3
+
+
 @pytest.mark.parametrize("N", [8, 16, 48])
 def test_lagrange_tables_bitwise_equal_reference(N):
     w, k = lagrange_tables(Grid(N))
