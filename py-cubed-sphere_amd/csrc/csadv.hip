@@ -795,13 +795,16 @@ __global__ void k_wind_avg(Solve *solves) {
     return acc;
   };
   const int ne = N + 1;   // edges i0 .. iend
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < 2 * ne * M; idx += gridDim.x * blockDim.x) {
-    const bool xdir = idx < ne * M;
-    const int id2 = xdir ? idx : idx - ne * M;
-    // consecutive threads take consecutive addresses: the edge index runs fastest on x-edges
-    // ([row][edge]), the column on y-edges ([edge][column])
-    const int e = (xdir ? id2 % ne : id2 / M) + i0;   // edge index along the flux direction
-    const int c = xdir ? id2 / ne : id2 % M;          // row (x-edges) / column (y-edges)
+  // one block per row of edges: x-edge rows c = 0 .. M-1 ([row][edge], edges i0 .. iend), then
+  // y-edge rows e = i0 .. iend ([edge][column], columns 0 .. M-1); threads run along the row,
+  // so consecutive threads take consecutive addresses and no index is divided
+  const int r = blockIdx.x;
+  if (r >= M + ne) return;                    // the grid covers the largest solve
+  const bool xdir = r < M;
+  const int len = xdir ? ne : M;
+  for (int t = threadIdx.x; t < len; t += blockDim.x) {
+    const int e = xdir ? t + i0 : r - M + i0;   // edge index along the flux direction
+    const int c = xdir ? r : t;                 // row (x-edges) / column (y-edges)
     const int64_t bs = xdir ? bs_u : bs_v;
     const double *W = xdir ? Wu : Wv;
     const int64_t off = xdir ? (int64_t)c * ldu + e : (int64_t)e * ld + c;
@@ -1087,7 +1090,7 @@ int wind_avg(csadv_t *h, hipStream_t st, bool all) {
   if (!any) return CSADV_OK;
   int maxN = 0;
   for (auto &d : h->desc) maxN = std::max(maxN, d.N);
-  const int blocks = std::min(1024, ((maxN + 1) * (maxN + 8) * 2 + BLOCK - 1) / BLOCK);
+  const int blocks = (maxN + 2 * NG) + (maxN + 1);   // x-edge rows, then y-edge rows
   if (h->dtype == CSADV_F64)
     hipLaunchKernelGGL(k_wind_avg<double>, dim3(blocks, 6, ns), dim3(BLOCK), 0, st, h->dev);
   else
