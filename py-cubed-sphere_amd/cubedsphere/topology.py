@@ -137,11 +137,16 @@ def ghost_recipes(grid, ld, depth=3):
     pstride = M * ld
     ks = np.arange(4, N + 4)
     # ---- strips
+    # Order within one (panel, side): when the source strip runs along x in the neighbour
+    # (ir != 0), the along-edge index is fastest, so consecutive cells read consecutive
+    # addresses; when it runs down a column, the depth is fastest, so consecutive cells read
+    # (and, on the east and west sides, write) the neighbouring cells of one row.
     tgt, src, wts, gsrc = [], [], [], []
     strip_of = {}
     for p in range(6):
         for side in (E, W, N_, S):
             q, ib, id_, ir, jb, jd, jr = (int(v) for v in orient[p, side])
+            parts = []
             for d in range(4):   # corner stencils may reach the 4th layer of a strip
                 K = kmin[d, ks][:, None] + np.arange(4)[None, :]
                 si = ib + id_ * d + ir * K
@@ -154,10 +159,18 @@ def ghost_recipes(grid, ld, depth=3):
                 for n in range(len(ks)):
                     strip_of[(p, int(ti[n]), int(tj[n]))] = (s_off[n], g_off[n], w[d, ks[n], :])
                 if d < depth:
-                    tgt.append(p * pstride + tj * ld + ti)
-                    src.append(s_off)
-                    gsrc.append(g_off)
-                    wts.append(w[d, ks, :])
+                    parts.append((p * pstride + tj * ld + ti, s_off, g_off, w[d, ks, :]))
+            if ir != 0:
+                for t_, s_, g_, w_ in parts:
+                    tgt.append(t_)
+                    src.append(s_)
+                    gsrc.append(g_)
+                    wts.append(w_)
+            else:
+                tgt.append(np.stack([t_ for t_, _, _, _ in parts], axis=1).reshape(-1))
+                src.append(np.stack([s_ for _, s_, _, _ in parts], axis=1).reshape(-1, 4))
+                gsrc.append(np.stack([g_ for _, _, g_, _ in parts], axis=1).reshape(-1, 4))
+                wts.append(np.stack([w_ for _, _, _, w_ in parts], axis=1).reshape(-1, 4))
     out = {
         "strip_tgt": np.concatenate(tgt).astype(np.int32),
         "strip_src": np.concatenate(src).astype(np.int32),
