@@ -276,8 +276,10 @@ __global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
 // The windows are rings indexed by the iteration's phase modulo 6 (the loop is unrolled by
 // six, so every slot is a fixed register and nothing is shifted); row quantities that the
 // column stage needs four iterations later (metric, 1/g, wind) ride in the same rings.
-// The inputs of row y+PF are requested at the end of iteration y into one of PF = 2 rotating
-// buffers, so their memory latency overlaps a whole iteration of arithmetic.
+// The inputs of row y+PF are requested at the end of iteration y, PF = 2 rows ahead, so their
+// memory latency overlaps a whole iteration of arithmetic: the row's state, correction metric
+// and y-edge wind into one of two rotating buffers, the step-invariant tables and winds
+// straight into their ring slots.
 // (src/discrete_operators.py:18-101, src/flux.py, src/reconstruction_1d.py,
 // src/advection_timestep.py:43)
 template <typename T>
@@ -294,8 +296,8 @@ struct StepIn {     // inputs of iteration y at the lane's column: row y and y-e
 template <typename T, int RECON, bool RK2, int SPLIT>
 struct March {
   static constexpr int split = SPLIT;
-  // rows whose inputs are in flight ahead of the one being processed (a third row measured no
-  // faster at N=384 FP64 and costs 18-28 VGPRs)
+  // rows whose inputs are in flight ahead of the one being processed (a third row of state
+  // measured slower: its extra row offsets cost more than the longer lead saves)
   static constexpr int PF = 2;
   static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
