@@ -293,7 +293,7 @@ struct StepIn {     // inputs of iteration y at the lane's column: row y and y-e
 
 // SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07); the metric treatment follows from it, since
 // the valid pairs are (AVLT or L04, MT-0) and (PL07, MT-PL07) (src/discrete_operators.py:48-82).
-template <typename T, int RECON, bool RK2, int SPLIT>
+template <typename T, int RECON, bool RK2, int SPLIT, bool WIND = false>
 struct March {
   static constexpr int split = SPLIT;
   // rows whose inputs are in flight ahead of the one being processed (a third row of state
@@ -302,6 +302,10 @@ struct March {
   static constexpr bool mt0 = SPLIT != 3;
   // ---- inputs (per wavefront and tracer)
   const T *Q, *Qa, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
+  // WIND: the winds of the step are formed here from the one-field FP64 basis (WU, WV) and its
+  // coefficient ws at the step's time, as k_wind_avg forms them for RK1, instead of read
+  const double *WU, *WV;
+  double ws;
   T *Qo;
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
   bool out_lane, need_col;
@@ -325,7 +329,8 @@ struct March {
     static_assert(PF == 2, "the ring slots are reused two rows ahead");
     const int yc = min(y, M - 1);
     const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
-    uw[SLOT] = ld_at(U, ou);
+    if constexpr (WIND) uw[SLOT] = T(wind_of_basis(ws, ld_at(WU, ou)));
+    else uw[SLOT] = ld_at(U, ou);
     if constexpr (RK2) usw[SLOT] = ld_at(US, ou);
     guw[SLOT] = ld_at(gu, ou);
     gcw[SLOT] = ld_at(gc, o);
@@ -351,7 +356,8 @@ struct March {
       }
     }
     const int ec = min(max(y - 3, 0), M - 1);
-    r.v = ld_at(V, (uint32_t)(ec * ld + xc));
+    if constexpr (WIND) r.v = T(wind_of_basis(ws, ld_at(WV, (uint32_t)(ec * ld + xc))));
+    else r.v = ld_at(V, (uint32_t)(ec * ld + xc));
     r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
     gvr[SLOT] = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
   }
@@ -489,7 +495,7 @@ struct March {
   }
 };
 
-template <typename T, int RECON, bool RK2, int SPLIT>
+template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int lane = threadIdx.x;
   const csadv_solve_desc &D = S.d;
@@ -500,7 +506,7 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
   const int item = (p * S.segs + wk.w) * S.strips + wk.z;
   const bool mfpr = D.mf == 3;
-  March<T, RECON, RK2, SPLIT> m;
+  March<T, RECON, RK2, SPLIT, WIND> m;
   m.ld = D.ld;
   m.ldu = D.ldu;
   m.M = S.M;
@@ -527,11 +533,22 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int64_t off = tr * S.tstride + p * S.pstride;
   m.Q = reinterpret_cast<const T *>(D.q0) + off;
   m.Qa = reinterpret_cast<const T *>(D.q1) + off;
-  StepIn<T> b[March<T, RECON, RK2, SPLIT>::PF];
-  m.start(b);
-  // read after the first rows are requested: a wait for any scalar load waits for all of them
-  // (they return out of order), so reading these first would hold back the rows' loads
-  const int step = D.ctl[2];
+  StepIn<T> b[March<T, RECON, RK2, SPLIT, WIND>::PF];
+  int step;
+  if constexpr (WIND) {
+    // the winds of the first rows need the step's coefficient: the step counter comes first
+    step = D.ctl[2];
+    const int row = min(step, D.max_steps - 1);     // as k_wind_avg (one basis field)
+    m.ws = D.coef[row * 2];
+    m.WU = D.wbu + p * S.pstride_u;
+    m.WV = D.wbv + p * S.pstride_v;
+    m.start(b);
+  } else {
+    m.start(b);
+    // read after the first rows are requested: a wait for any scalar load waits for all of them
+    // (they return out of order), so reading these first would hold back the rows' loads
+    step = D.ctl[2];
+  }
   const double c0 = mfpr ? D.corr[tr] : 0.0, c1 = mfpr ? D.corr[D.ntr + tr] : 0.0;
   if (D.stop > 0 && step >= D.stop) return;   // solve finished
   const bool odd = step & 1;                  // the state of step n lives in q[n & 1]
@@ -549,11 +566,11 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   }
 }
 
-template <typename T, int RECON, bool RK2, int SPLIT>
+template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves, const int4 *__restrict__ work) {
   const int4 wk = work[blockIdx.x];
   if (wk.x < 0) return;              // padding item of the XCD-banded work list
-  step_item<T, RECON, RK2, SPLIT>(solves[wk.x], wk);
+  step_item<T, RECON, RK2, SPLIT, WIND>(solves[wk.x], wk);
 }
 
 // Work item b of a single-solve handle, computed instead of read: the list the host builds
@@ -572,11 +589,11 @@ __device__ __forceinline__ bool single_item(const Solve &S, int b, int4 &wk) {
 
 // A handle of one solve: the descriptor travels in the kernel arguments and the work item is
 // computed, so the first loads of a wavefront depend on nothing but the arguments.
-template <typename T, int RECON, bool RK2, int SPLIT>
+template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step1(const Solve one) {
   int4 wk;
   if (!single_item(one, blockIdx.x, wk)) return;
-  step_item<T, RECON, RK2, SPLIT>(one, wk);
+  step_item<T, RECON, RK2, SPLIT, WIND>(one, wk);
 }
 
 // Fold the correction totals into the current buffer (interior only); k_clear_corr then
@@ -792,6 +809,7 @@ __global__ void k_wind_avg(Solve *solves) {
   T *us = D.usgn ? reinterpret_cast<T *>(D.usgn) + p * S.pstride_u : nullptr;
   T *vs = D.vsgn ? reinterpret_cast<T *>(D.vsgn) + p * S.pstride_v : nullptr;
   auto comb = [&](const double *W, int64_t bs, int64_t off, const double *s) {
+    if (K == 1) return wind_of_basis(s[0], W[off]);   // the same bits as the step's own (WIND)
     double acc = 0.0;
     for (int m = 0; m < K; ++m) acc += s[m] * W[m * bs + off];
     return acc;
@@ -812,12 +830,12 @@ __global__ void k_wind_avg(Solve *solves) {
     const int64_t off = xdir ? (int64_t)c * ldu + e : (int64_t)e * ld + c;
     const int64_t step = xdir ? 1 : ld;
     const double rd = xdir ? 1.0 / D.dx : 1.0 / D.dy;
-    const double u = (K == 1 && s1[0] == 1.0) ? W[off] : comb(W, bs, off, s1);
+    const double u = comb(W, bs, off, s1);
     double avg = u;
     if (rk2) {
       auto ui = [&](int64_t o) {
-        const double un = (K == 1 && s1[0] == 1.0) ? W[o] : comb(W, bs, o, s1);
-        const double uo = (K == 1 && s0[0] == 1.0) ? W[o] : comb(W, bs, o, s0);
+        const double un = comb(W, bs, o, s1);
+        const double uo = comb(W, bs, o, s0);
         return 1.5 * un - 0.5 * uo;
       };
       const double a = (u * dto2) * rd;
@@ -918,6 +936,8 @@ struct csadv_s {
   bool rk2 = false;               // dp = 2 (separate upwind-side winds)
   int split = 1;                  // splitting of every solve (its metric treatment follows)
   bool general = false;           // ET-S72 / ET-PL07 / MF-AF: the general step
+  bool wind1 = false;             // every solve RK1 with a one-field time-dependent wind: the
+                                  // step kernel forms its winds (no per-step k_wind_avg)
   double *red = nullptr;          // small reduction scratch (mass / norms)
   std::vector<int> done;          // host copy of the steps completed per solve (ctl[1])
   std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
@@ -1018,12 +1038,17 @@ Solve derive(const csadv_solve_desc &d, int tile) {
 template <typename T, int R, int SP>
 hipError_t launch_step_s(csadv_t *h, hipStream_t st) {
   const bool one = h->host.size() == 1;
+  if (h->wind1) {                    // RK1 only
+    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP, true>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
+    else hipLaunchKernelGGL((k_step<T, R, false, SP, true>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+    return hipGetLastError();
+  }
   if (h->rk2) {
-    if (one) hipLaunchKernelGGL((k_step1<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
-    else hipLaunchKernelGGL((k_step<T, R, true, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+    if (one) hipLaunchKernelGGL((k_step1<T, R, true, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
+    else hipLaunchKernelGGL((k_step<T, R, true, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
   } else {
-    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
-    else hipLaunchKernelGGL((k_step<T, R, false, SP>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
+    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
+    else hipLaunchKernelGGL((k_step<T, R, false, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
   }
   return hipGetLastError();
 }
@@ -1223,6 +1248,9 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
     HIPCHK(h, hipMalloc(&h->work, sizeof(int4) * work.size()));
     h->nwork = (int)work.size();
   }
+  h->wind1 = !h->general;
+  for (int s = 0; s < nsolve; ++s)
+    h->wind1 = h->wind1 && solves[s].unsteady && solves[s].dp == 1 && solves[s].nbasis == 1;
   HIPCHK(h, hipMemcpyAsync(h->work, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice, st));
   HIPCHK(h, hipMemcpyAsync(h->dev, h->host.data(), sizeof(Solve) * nsolve, hipMemcpyHostToDevice, st));
   HIPCHK(h, hipMemcpyAsync(h->qsets, sets.data(), sizeof(FieldSet) * nsolve, hipMemcpyHostToDevice, st));
@@ -1386,7 +1414,7 @@ int enqueue_step(csadv_t *h, hipStream_t st, bool allow_prof) {
     if (rc) return rc;
     return h->dtype == CSADV_F64 ? enqueue_general_t<double>(h, st) : enqueue_general_t<float>(h, st);
   }
-  int rc = wind_avg(h, st, false);
+  int rc = h->wind1 ? CSADV_OK : wind_avg(h, st, false);
   if (rc) return rc;
   rc = step_ghosts(h, st);
   if (rc) return rc;

@@ -201,6 +201,12 @@ __device__ __forceinline__ void st_at(T *base, uint32_t idx, T v) {
   *(T CSADV_GLOBAL *)(b + idx * (uint32_t)sizeof(T)) = v;
 }
 
+// The wind of one edge from a one-field basis W and its coefficient s at the step's time
+// (src/averaged_velocity.py:14-62 with RK1: the wind at t_n), in FP64, in one expression shared by
+// the wind kernel and the step kernel so that both give the same bits (s = 1 gives w, with a
+// zero of either sign as +0).
+__device__ __forceinline__ double wind_of_basis(double s, double w) { return fma(s, w, 0.0); }
+
 // Split update of one cell after the first directional pass (src/discrete_operators.py:49-73).
 //   split 1 AVLT: (gQ + dF/2)/g ; 2 L04: (gQ + dF/2 + (c1-c2) Q/2)/g ; 3 PL07: (Q + (Q+dF)/(1-(c1-c2)))/2
 template <typename T>

@@ -327,7 +327,9 @@ class Advection:
         _lib.check(self.lib.csadv_ghost_fill(self.handle, self._stream()), self.handle, "csadv_ghost_fill")
 
     def winds(self):
-        """Time-averaged winds of the next step: (uavg [6][M][M+1], vavg [6][M+1][M]) numpy."""
+        """Time-averaged winds of the next step: (uavg [6][M][M+1], vavg [6][M+1][M]) numpy.
+        A handle whose solves are all RK1 with a one-field time-dependent wind forms the winds
+        inside the step kernel: its arrays then keep the winds of the first step."""
         M = self.grid.M
         return self.uavg[:, :, :M + 1].double().cpu().numpy(), self.vavg[:, :, :M].double().cpu().numpy()
 
