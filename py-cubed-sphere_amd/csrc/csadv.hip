@@ -789,12 +789,13 @@ __global__ void k_wind_c2e(const Solve *solves, int s) {
 // u(t) = sum_m s_m(t) W_m; RK1: u_avg = u(t_{n-1}); RK2: upwind-interpolated extrapolation.
 // Writes dtype uavg / usgn on x-edges e in [i0-1, iend+1] (all rows) and likewise for v.
 template <typename T>
-__global__ void k_wind_avg(Solve *solves) {
+__global__ void k_wind_avg(Solve *solves, int all) {
   Solve &S = solves[blockIdx.z];
   const csadv_solve_desc &D = S.d;
   const int N = D.N, M = S.M, ld = D.ld, ldu = D.ldu, K = D.nbasis;
   const int p = blockIdx.y;
   const int i0 = NG, iend = N + NG;
+  if (!all && !D.unsteady) return;                                  // steady: set at init
   if (D.stop > 0 && D.ctl[1] >= D.stop) return;                    // solve finished
   const int row = D.unsteady ? min(D.ctl[1], D.max_steps - 1) : 0;   // steps completed
   const double *s1 = D.coef + (int64_t)row * 2 * K;   // coefficients at t_{n-1}
@@ -1119,9 +1120,9 @@ int wind_avg(csadv_t *h, hipStream_t st, bool all) {
   for (auto &d : h->desc) maxN = std::max(maxN, d.N);
   const int blocks = (maxN + 2 * NG) + (maxN + 1);   // x-edge rows, then y-edge rows
   if (h->dtype == CSADV_F64)
-    hipLaunchKernelGGL(k_wind_avg<double>, dim3(blocks, 6, ns), dim3(BLOCK), 0, st, h->dev);
+    hipLaunchKernelGGL(k_wind_avg<double>, dim3(blocks, 6, ns), dim3(BLOCK), 0, st, h->dev, (int)all);
   else
-    hipLaunchKernelGGL(k_wind_avg<float>, dim3(blocks, 6, ns), dim3(BLOCK), 0, st, h->dev);
+    hipLaunchKernelGGL(k_wind_avg<float>, dim3(blocks, 6, ns), dim3(BLOCK), 0, st, h->dev, (int)all);
   HIPCHK(h, hipGetLastError());
   return CSADV_OK;
 }
