@@ -293,7 +293,7 @@ struct StepIn {     // inputs of iteration y at the lane's column: row y and y-e
 
 // SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07); the metric treatment follows from it, since
 // the valid pairs are (AVLT or L04, MT-0) and (PL07, MT-PL07) (src/discrete_operators.py:48-82).
-template <typename T, int RECON, bool RK2, int SPLIT, bool WIND = false>
+template <typename T, int RECON, bool RK2, int SPLIT, bool WIND = false, bool XPULL = false>
 struct March {
   static constexpr int split = SPLIT;
   // rows whose inputs are in flight ahead of the one being processed (a third row of state
@@ -308,6 +308,7 @@ struct March {
   double ws;
   T *Qo;
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
+  int am2, am1, ap1, ap2;    // byte addresses of the lanes -2, -1, +1, +2 (mod 64) for lane pulls
   bool out_lane, need_col;
   T dtdx, dtdy;
   double cf;
@@ -411,8 +412,8 @@ struct March {
           const T ur = uw[s2];
           const bool urp = wind_sign(s2) >= T(0);
           const T gur = guw[s2];
-          const T qy_m1 = from_left(qy), qy_p1 = from_right(qy);
-          const T qy_m2 = from_left(qy_m1), qy_p2 = from_right(qy_p1);
+          T qy_m2, qy_m1, qy_p1, qy_p2;
+          xwindow<T, XPULL>(qy, am2, am1, ap1, ap2, qy_m2, qy_m1, qy_p1, qy_p2);
           const Parab<T> c = parabola<T, RECON>(qy_m2, qy_m1, qy, qy_p1, qy_p2, gur, from_right(gur), gcm, mt0);
           const T f2 = upwind_x<T>(urp, c, ur * dtdx, gur, from_right(gur), mt0);
           const T dF2 = f2 - from_right(f2);
@@ -439,8 +440,8 @@ struct March {
     const T q = in.q, u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
     const bool up = wind_sign(s0) >= T(0);
     const T gup1 = from_right(gu0);
-    const T q_m1 = from_left(q), q_p1 = from_right(q);
-    const T q_m2 = from_left(q_m1), q_p2 = from_right(q_p1);
+    T q_m2, q_m1, q_p1, q_p2;
+    xwindow<T, XPULL>(q, am2, am1, ap1, ap2, q_m2, q_m1, q_p1, q_p2);
     const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
     const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);
     const T dF1 = f1 - from_right(f1);
@@ -495,7 +496,7 @@ struct March {
   }
 };
 
-template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
+template <typename T, int RECON, bool RK2, int SPLIT, bool WIND, bool XPULL>
 __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int lane = threadIdx.x;
   const csadv_solve_desc &D = S.d;
@@ -506,7 +507,7 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
   const int item = (p * S.segs + wk.w) * S.strips + wk.z;
   const bool mfpr = D.mf == 3;
-  March<T, RECON, RK2, SPLIT, WIND> m;
+  March<T, RECON, RK2, SPLIT, WIND, XPULL> m;
   m.ld = D.ld;
   m.ldu = D.ldu;
   m.M = S.M;
@@ -516,6 +517,10 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.xe = min(m.x, m.M);
   m.tj = tj;
   m.ylast = tj + ny + 3;
+  m.am2 = ((lane + 62) & 63) << 2;
+  m.am1 = ((lane + 63) & 63) << 2;
+  m.ap1 = ((lane + 1) & 63) << 2;
+  m.ap2 = ((lane + 2) & 63) << 2;
   m.out_lane = lane >= 3 && lane < 3 + nx;
   m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
   m.dtdx = T(D.dt / D.dx);
@@ -533,7 +538,7 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int64_t off = tr * S.tstride + p * S.pstride;
   m.Q = reinterpret_cast<const T *>(D.q0) + off;
   m.Qa = reinterpret_cast<const T *>(D.q1) + off;
-  StepIn<T> b[March<T, RECON, RK2, SPLIT, WIND>::PF];
+  StepIn<T> b[March<T, RECON, RK2, SPLIT, WIND, XPULL>::PF];
   int step;
   if constexpr (WIND) {
     // the winds of the first rows need the step's coefficient: the step counter comes first
@@ -570,7 +575,7 @@ template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves, const int4 *__restrict__ work) {
   const int4 wk = work[blockIdx.x];
   if (wk.x < 0) return;              // padding item of the XCD-banded work list
-  step_item<T, RECON, RK2, SPLIT, WIND>(solves[wk.x], wk);
+  step_item<T, RECON, RK2, SPLIT, WIND, false>(solves[wk.x], wk);
 }
 
 // Work item b of a single-solve handle, computed instead of read: the list the host builds
@@ -589,11 +594,13 @@ __device__ __forceinline__ bool single_item(const Solve &S, int b, int4 &wk) {
 
 // A handle of one solve: the descriptor travels in the kernel arguments and the work item is
 // computed, so the first loads of a wavefront depend on nothing but the arguments.
+// FP32 takes its x windows through the crossbar (measured 3% faster at N=768 with 4 tracers);
+// FP64 and the multi-solve kernel keep the DPP shifts (measured 0.5-4% slower with the crossbar).
 template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step1(const Solve one) {
   int4 wk;
   if (!single_item(one, blockIdx.x, wk)) return;
-  step_item<T, RECON, RK2, SPLIT, WIND>(one, wk);
+  step_item<T, RECON, RK2, SPLIT, WIND, std::is_same<T, float>::value>(one, wk);
 }
 
 // Fold the correction totals into the current buffer (interior only); k_clear_corr then

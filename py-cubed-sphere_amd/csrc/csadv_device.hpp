@@ -168,6 +168,34 @@ __device__ __forceinline__ double dpp_move(double v) {
 template <typename T> __device__ __forceinline__ T from_left(T v) { return dpp_move<kDppWaveShr1>(v); }
 template <typename T> __device__ __forceinline__ T from_right(T v) { return dpp_move<kDppWaveShl1>(v); }
 
+// Lane pull through the LDS crossbar (ds_bpermute_b32: no LDS memory, no VALU slot): each lane
+// reads v of the lane whose byte address (4 * lane) it passes; the address wraps modulo 64 lanes.
+__device__ __forceinline__ float lane_pull(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_pull(double v, int addr) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)b);
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The 5-cell window of every lane's cell along the wavefront (x): neighbours at -2, -1, +1, +2.
+// PULL: through the crossbar (am2 / am1 / ap1 / ap2: byte addresses of the lanes two and one to
+// the left and right, mod 64), which frees four VALU slots per window and adds the crossbar's
+// latency; else by DPP shifts.  End lanes read zeros (DPP) or wrapped lanes (crossbar); no
+// stored cell's stencil reaches them.
+template <typename T, bool PULL>
+__device__ __forceinline__ void xwindow(T v, int am2, int am1, int ap1, int ap2, T &m2, T &m1, T &p1, T &p2) {
+  if constexpr (PULL) {
+    m2 = lane_pull(v, am2); m1 = lane_pull(v, am1);
+    p1 = lane_pull(v, ap1); p2 = lane_pull(v, ap2);
+  } else {
+    m1 = from_left(v); p1 = from_right(v);
+    m2 = from_left(m1); p2 = from_right(p1);
+  }
+}
+
 // Flux through the left edge of each lane's cell along the wavefront (x), lane = cell: every
 // lane forms the flux its cell sends through its right edge when it is the upwind cell there
 // (c_r = c of the right neighbour) and through its left edge when it is the upwind cell there;
