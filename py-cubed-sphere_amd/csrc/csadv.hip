@@ -136,20 +136,10 @@ __device__ __forceinline__ double block_sum_fixed(double v, double *red) {
   return s;
 }
 
-// Mass-fixer correction total C after `steps` steps.  With the partials of the last step
-// pending (pending = ctl[3], read by the caller up front), C = C(steps - 1) + dt m0 / a2,
-// m0 = sum of div g over the interior (src/discrete_operators.py:98-101) summed in a fixed
-// order so that every block (and every run) gets the same bits; otherwise C is the stored
-// total of this step.  The partial sums are requested whether or not they are pending, so
-// that no load waits on the flag; every thread issues its eight loads of a batch before
-// adding them in the order i, i+BLOCK, ... (masked past the end).
-__device__ double pending_corr(const Solve &S, int tr, double *red, int pending, int steps) {
-  const csadv_solve_desc &D = S.d;
-  if (D.mf != 3) return 0.0;
-  const double held = D.corr[(steps & 1) * D.ntr + tr];
-  const double prev = D.corr[((steps + 1) & 1) * D.ntr + tr];
-  const double *P = D.partials + (int64_t)tr * S.part_per_tracer;
-  const int n = S.part_per_tracer;
+// Sum of a tracer's mass-fixer partials (the last step's div g per wavefront), in a fixed
+// order so that every run gets the same bits: every thread issues its eight loads of a batch
+// before adding them in the order i, i+BLOCK, ... (masked past the end), then block_sum_fixed.
+__device__ double partial_sum(const double *P, int n, double *red) {
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += 8 * BLOCK) {
     double v[8];
@@ -159,29 +149,63 @@ __device__ double pending_corr(const Solve &S, int tr, double *red, int pending,
     for (int u = 0; u < 8; ++u)
       if (i + u * BLOCK < n) acc += v[u];
   }
-  const double sum = block_sum_fixed(acc, red);
-  return pending ? prev + D.dt * sum / D.a2 : held;
+  return block_sum_fixed(acc, red);
+}
+
+// Mass-fixer correction total C after `steps` steps, from the stored totals c0 / c1 (slots of
+// even / odd steps) and the partials' sum: with the partials of the last step pending,
+// C = C(steps - 1) + dt m0 / a2, m0 = sum of div g over the interior
+// (src/discrete_operators.py:98-101); otherwise C is the stored total of this step.
+__device__ __forceinline__ double corr_total(double dt, double a2, double c0, double c1, double sum, int pending,
+                                             int steps) {
+  const double held = (steps & 1) ? c1 : c0, prev = (steps & 1) ? c0 : c1;
+  return pending ? prev + dt * sum / a2 : held;
 }
 
 template <typename T>
 __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
   const csadv_solve_desc &D = S.d;
-  const int tr = blockIdx.z;           // one tracer per grid layer: tracers run side by side
-  if (tr >= D.ntr) return;
-  if (blockIdx.x == gridDim.x - 1) {   // the correction total after the last step
-    if (D.mf != 3) return;
-    const int steps = D.ctl[1], pending = D.ctl[3];
-    if (D.stop > 0 && steps >= D.stop) return;
-    const double cf = pending_corr(S, tr, red, pending, steps);
-    if (threadIdx.x == 0 && pending) D.corr[(steps & 1) * D.ntr + tr] = cf;
+  // Every descriptor field of this kernel is requested up front: the empty asm below takes them
+  // all, so their scalar loads are issued together before the first branch (one round trip,
+  // where loading each field at its first use took four dependent ones).  The asm is neither
+  // volatile nor a memory clobber, so the loads after it may still be scalar loads.
+  const int ntr = D.ntr, mf = D.mf, stop = D.stop, n_strip = D.n_strip, n_corner = D.n_corner;
+  const int part_n = S.part_per_tracer;
+  const int64_t tstride = S.tstride;
+  const double dt = D.dt, a2 = D.a2;
+  int32_t *const ctl = D.ctl;
+  double *const corr = D.corr;
+  const double *const partials = D.partials;
+  void *const q0 = D.q0, *const q1 = D.q1;
+  const int32_t *const strip_src = D.strip_src, *const strip_tgt = D.strip_tgt;
+  const int32_t *const corner_src = D.corner_src, *const corner_tgt = D.corner_tgt;
+  const double *const strip_w = D.strip_w, *const corner_w = D.corner_w;
+  const unsigned last_block = gridDim.x - 1;
+  // one tracer per grid layer (tracers run side by side); the tracer index passes through the
+  // asm (unchanged), so it exists only once every field is in registers
+  int tr = blockIdx.z;
+  asm("" : "=s"(tr) : "0"(tr), "s"(ntr), "s"(mf), "s"(stop), "s"(n_strip), "s"(n_corner), "s"(part_n),
+      "s"(tstride), "s"(dt), "s"(a2), "s"(ctl), "s"(corr), "s"(partials), "s"(q0), "s"(q1), "s"(strip_src),
+      "s"(strip_tgt), "s"(corner_src), "s"(corner_tgt), "s"(strip_w), "s"(corner_w), "s"(last_block));
+  if (tr >= ntr) return;
+  const int steps = ctl[1];            // requested before the recipes, whose wait it would join
+  if (blockIdx.x == last_block) {      // the correction total after the last step
+    if (mf != 3) return;
+    // the partials and both stored totals are requested with the control words, not after
+    // them: no load of this block waits for another
+    const double c0 = corr[tr], c1 = corr[ntr + tr];
+    const int pending = ctl[3];
+    const double sum = partial_sum(partials + (int64_t)tr * part_n, part_n, red);
+    if (stop > 0 && steps >= stop) return;
+    if (threadIdx.x == 0 && pending) corr[(steps & 1) * ntr + tr] = corr_total(dt, a2, c0, c1, sum, 1, steps);
     return;
   }
-  const int ncell = D.n_strip + D.n_corner;
+  const int ncell = n_strip + n_corner;
   if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
   const int idx = blockIdx.x * BLOCK + threadIdx.x;
   const bool active = idx < ncell;
-  const bool strip = idx < D.n_strip;
-  const int c = strip ? idx : idx - D.n_strip;
+  const bool strip = idx < n_strip;
+  const int c = strip ? idx : idx - n_strip;
   // the recipe of this cell does not depend on the step: request all of it first, so that
   // its latency overlaps the read of the step counter
   int s16[16];
@@ -189,34 +213,33 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
   int tgt = 0;
   if (active) {
     if (strip) {
-      const int4 s4 = reinterpret_cast<const int4 *>(D.strip_src)[c];
+      const int4 s4 = reinterpret_cast<const int4 *>(strip_src)[c];
       s16[0] = s4.x; s16[1] = s4.y; s16[2] = s4.z; s16[3] = s4.w;
-      const double2 *w2 = reinterpret_cast<const double2 *>(D.strip_w + 4 * c);
+      const double2 *w2 = reinterpret_cast<const double2 *>(strip_w + 4 * c);
       const double2 a = w2[0], b = w2[1];
       w16[0] = a.x; w16[1] = a.y; w16[2] = b.x; w16[3] = b.y;
-      tgt = D.strip_tgt[c];
+      tgt = strip_tgt[c];
     } else {
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
-        const int4 s4 = reinterpret_cast<const int4 *>(D.corner_src + 16 * c)[l];
+        const int4 s4 = reinterpret_cast<const int4 *>(corner_src + 16 * c)[l];
         s16[4 * l] = s4.x; s16[4 * l + 1] = s4.y; s16[4 * l + 2] = s4.z; s16[4 * l + 3] = s4.w;
       }
-      const double2 *w2 = reinterpret_cast<const double2 *>(D.corner_w + 16 * c);
+      const double2 *w2 = reinterpret_cast<const double2 *>(corner_w + 16 * c);
 #pragma unroll
       for (int l = 0; l < 8; ++l) {
         const double2 a = w2[l];
         w16[2 * l] = a.x; w16[2 * l + 1] = a.y;
       }
-      tgt = D.corner_tgt[c];
+      tgt = corner_tgt[c];
     }
   }
-  const int steps = D.ctl[1];
   const bool lead = blockIdx.x == 0 && tr == 0 && threadIdx.x == 0;
-  if (D.stop > 0 && steps >= D.stop) {      // solve finished: k_step reads ctl[2] and skips it too
-    if (lead) D.ctl[2] = steps;
+  if (stop > 0 && steps >= stop) {          // solve finished: k_step reads ctl[2] and skips it too
+    if (lead) ctl[2] = steps;
     return;
   }
-  T *Qt = reinterpret_cast<T *>((steps & 1) ? D.q1 : D.q0) + tr * S.tstride;
+  T *Qt = reinterpret_cast<T *>((steps & 1) ? q1 : q0) + tr * tstride;
   double acc = 0.0;                         // in recipe order
   if (active) {
     if (strip) {
@@ -234,7 +257,7 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
     }
     Qt[tgt] = (T)acc;
   }
-  if (lead) D.ctl[2] = steps;
+  if (lead) ctl[2] = steps;
 }
 
 // Add the last step's partials to the correction total (once; used by finalize and the
@@ -246,7 +269,8 @@ __global__ void k_settle(Solve *solves, int first) {
   if (D.mf != 3 || !D.ctl[3]) return;
   const int steps = D.ctl[1];
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const double cf = pending_corr(S, tr, red, 1, steps);
+    const double sum = partial_sum(D.partials + (int64_t)tr * S.part_per_tracer, S.part_per_tracer, red);
+    const double cf = corr_total(D.dt, D.a2, D.corr[tr], D.corr[D.ntr + tr], sum, 1, steps);
     if (threadIdx.x == 0) D.corr[(steps & 1) * D.ntr + tr] = cf;
   }
   __syncthreads();
@@ -571,6 +595,24 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   }
 }
 
+// Bring every cache line of the kernel-argument descriptor into the scalar cache with loads
+// issued together: an empty asm takes one word of each line and passes v through unchanged, so
+// v exists only once all of them have arrived (one scalar round trip); the field loads at their
+// first use then hit the cache instead of waiting for their lines one after another (the
+// one-solve step kernel's start took four such dependent round trips).  The asm is neither
+// volatile nor a memory clobber, so the loads after it may still be scalar loads.  (For a
+// descriptor in global memory the compiler reads the words with vector loads, which warm no
+// scalar cache: the multi-solve kernel does not use it.)
+__device__ __forceinline__ int touch_solve(const Solve &S, int v) {
+  static_assert(sizeof(Solve) == 424, "one word in each 64-byte line of the descriptor");
+  const char *b = reinterpret_cast<const char *>(&S);
+  int w[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) __builtin_memcpy(&w[k], b + (k < 7 ? 64 * k : 420), 4);
+  asm("" : "=s"(v) : "0"(v), "s"(w[0]), "s"(w[1]), "s"(w[2]), "s"(w[3]), "s"(w[4]), "s"(w[5]), "s"(w[6]), "s"(w[7]));
+  return v;
+}
+
 template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves, const int4 *__restrict__ work) {
   const int4 wk = work[blockIdx.x];
@@ -599,7 +641,7 @@ __device__ __forceinline__ bool single_item(const Solve &S, int b, int4 &wk) {
 template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step1(const Solve one) {
   int4 wk;
-  if (!single_item(one, blockIdx.x, wk)) return;
+  if (!single_item(one, touch_solve(one, blockIdx.x), wk)) return;
   step_item<T, RECON, RK2, SPLIT, WIND, std::is_same<T, float>::value>(one, wk);
 }
 
