@@ -14,6 +14,9 @@ Workloads (--workload):
                        one launch per kernel per step over all solves, graph-replayed,
                        tile shape from the autotuner
   polar_n48_f64        the reference's CPU-sized case (N=48 polar rotation), for reference
+  two_hills_n384_f64_etpl07    the N=384 FP64 case through the general step with schemes 1
+  two_hills_n384_f64_rk2_mfaf  and 3 of the reference's convergence study (ET-PL07 with the
+                       PL07 split; RK2 with MF-AF); its roofline is over the whole step
 
 A "step" is one full advection step of every cell of every solve: ghost fill, both
 directional passes of the split, update and mass fixer (plus the wind update of
@@ -51,7 +54,34 @@ WORKLOADS = {
     "batched_64_f32": ("batched", None, None, "f32", 0.5, 200),
     "polar_n48_f64": ("single", 48, "solid_body_polar", "f64", 0.5, 384),
     "served_poisson_f32": ("served", None, None, "f32", 0.5, None),
+    # the general step (the options the fused step kernel does not cover), on the N=384 case
+    "two_hills_n384_f64_etpl07": ("single", 384, "two_hills_tilted", "f64", 0.5, 300),
+    "two_hills_n384_f64_rk2_mfaf": ("single", 384, "two_hills_tilted", "f64", 0.5, 300),
 }
+# par/advection.par option codes of a workload (default: recon 3, dp 1, split 1, et 3, mt 1, mf 3);
+# the two general-step workloads are schemes 1 and 3 of src/advection_error.py:61-66
+WORKLOAD_OPTS = {
+    "two_hills_n384_f64_etpl07": dict(opsplit=3, et=2, mt=2, mf=1),       # PPM-PL07, SP-PL07, ET-PL07, MT-PL07, MF-0
+    "two_hills_n384_f64_rk2_mfaf": dict(dp=2, opsplit=1, et=3, mt=1, mf=2),  # RK2, SP-AVLT, ET-DG, MT-0, MF-AF
+}
+OPTION_NAMES = {
+    "recon": ("PPM-0", "PPM-CW84", "PPM-PL07", "PPM-L04"), "dp": ("RK1", "RK2"),
+    "opsplit": ("AVLT", "L04", "PL07"), "et": ("ET-S72", "ET-PL07", "ET-DG"), "mt": ("MT-0", "MT-PL07"),
+    "mf": ("MF-0", "MF-AF", "MF-PR"),
+}
+
+
+def options_string(opts):
+    o = dict(recon=3, dp=1, opsplit=1, et=3, mt=1, mf=3)
+    o.update(opts)
+    n = {k: OPTION_NAMES[k][o[k] - 1] for k in OPTION_NAMES}
+    return "recon %s, dp %s, split %s, %s, %s, %s" % (n["recon"], n["dp"], n["opsplit"], n["et"], n["mt"], n["mf"])
+
+
+def is_general(opts):
+    """The general step runs every option pair the fused step kernel does not: ET-S72, ET-PL07
+    and MF-AF."""
+    return opts.get("et", 3) != 3 or opts.get("mf", 3) == 2
 SERVED_REQUESTS, SERVED_RATE = 200, 12.0      # Poisson stream: count, arrivals per second
 BATCH_NS = (32, 48, 96, 192, 384, 512)
 # tile ids from the autotuner for the single-solve workloads (the batched workload tunes
@@ -77,12 +107,12 @@ def traffic_from_profile(name):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(N, case, dt, seconds):
-    """Time the oracle (numpy FP64, single thread) on one solve of the workload: 1 warm-up
-    step, then as many steps as fit in about ``seconds``."""
+def cpu_baseline(N, case, dt, seconds, opts=None):
+    """Time the oracle (numpy FP64, single thread) on one solve of the workload, with the
+    workload's options: 1 warm-up step, then as many steps as fit in about ``seconds``."""
     from oracle import timestep as ots
     from oracle.cases import from_case
-    grid, sim = ots.make(N, from_case(case), dt)
+    grid, sim = ots.make(N, from_case(case), dt, **(opts or {}))
     ots.init_vars_adv(grid, sim)
     ots.adv_time_step(grid, sim, 1, dt)
     steps, t0 = 0, time.perf_counter()
@@ -182,6 +212,8 @@ def main():
     from cubedsphere.cases import workload
 
     kind, N, case_name, dtype, cfl, default_steps = WORKLOADS[args.workload]
+    opts = WORKLOAD_OPTS.get(args.workload, {})
+    general = is_general(opts)
     if kind == "served":
         print(json.dumps(run_served(args)))
         return
@@ -203,7 +235,7 @@ def main():
     tiles = {s[0]: tile for s in specs}
     # many solves: geometry, fields and winds prepared on the device
     device_grid = kind == "batched"
-    members = make_members(specs, dtype=dtype, cfl=cfl, tiles=tiles, device_grid=device_grid)
+    members = make_members(specs, dtype=dtype, cfl=cfl, tiles=tiles, device_grid=device_grid, **opts)
     stepper = Batch(members, graph_steps=0).init()
     if args.calibrate_copy_mib > 0:
         n = args.calibrate_copy_mib * (1 << 20) // 8
@@ -221,7 +253,15 @@ def main():
     b_alg = bytes_per_launch / per_step
     in_step_s = kern_ms / 1e3 / max(launches, 1)       # event pair around each launch inside the step
     burst_reps = 200
-    kern_s = stepper.time_step_kernel(burst_reps) * 1e-6   # one event pair around a graph of launches
+    if general:
+        # a sequence of simple kernels per step: the roofline is taken over the whole step
+        kern_s = elapsed / K
+        kernel, timing = "general step (its whole kernel sequence)", "wall time of the timed steps"
+    else:
+        kern_s = stepper.time_step_kernel(burst_reps) * 1e-6   # one event pair around a graph of launches
+        kernel = "k_step"
+        timing = ("HIP events on the launch stream around one graph of %d back-to-back step-kernel "
+                  "launches (each redoes the same step)" % burst_reps)
     achieved = bytes_per_launch / kern_s / 1e9
     Ns = sorted({m.grid.N for m in members})
 
@@ -241,7 +281,7 @@ def main():
         "config": {"workload": args.workload, "solves": len(members), "N": Ns if len(Ns) > 1 else Ns[0],
                    "cells_per_step": per_step, "tracers": sum(m.case.ntr for m in members), "cfl": cfl,
                    "tile": tile, "tile_rows": tile_rows(tile), "graph_steps": args.graph,
-                   "options": "recon PPM-PL07, dp RK1, split AVLT, ET-DG, MT-0, MF-PR"},
+                   "options": options_string(opts)},
         "hbm_frac_whole_step": value * b_alg / (HBM_PEAK_GBS * 1e9),
         "roofline": {
             "bound": "hbm",
@@ -249,14 +289,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(args.workload),
-            "kernel": "k_step",
+            "traffic": None if general else traffic_from_profile(args.workload),
+            "kernel": kernel,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "mean_launch_us": kern_s * 1e6,
-            "launches_timed": burst_reps,
-            "timing": "HIP events on the launch stream around one graph of %d back-to-back step-kernel "
-                      "launches (each redoes the same step)" % burst_reps,
-            "mean_launch_us_in_step": in_step_s * 1e6,
+            "launches_timed": K if general else burst_reps,
+            "timing": timing,
+            "mean_launch_us_in_step": None if general else in_step_s * 1e6,
         },
     }
     if tuned is not None:
@@ -273,7 +312,7 @@ def main():
         b64.close()
     if not args.no_cpu_baseline:
         m0 = members[0]
-        out["cpu_baseline"] = cpu_baseline(m0.grid.N, m0.case, m0.dt, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(m0.grid.N, m0.case, m0.dt, args.cpu_seconds, opts)
     print(json.dumps(out))
 
 

@@ -168,10 +168,13 @@ __global__ void k_gp_recon(const Solve *solves, int s, int tr, int dir, int fiel
   const unsigned M = v.M, nc = v.N + 2, total = 6u * M * nc;
   const int st = dir == 0 ? 1 : v.ld;
   for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const unsigned u = t / nc;
-    const int c = (int)(t % nc) + v.i0 - 1;          // cell along dir
-    const int a = u % M;                             // the other index
-    const int p = u / M;
+    // consecutive threads take consecutive columns in both directions (coalesced): along the
+    // row in x, across the columns in y
+    const unsigned w = dir == 0 ? nc : M;
+    const unsigned u = t / w, r = t % w;
+    const int c = (int)(dir == 0 ? r : u % nc) + v.i0 - 1;   // cell along dir
+    const int a = dir == 0 ? (int)(u % M) : (int)r;          // the other index
+    const int p = dir == 0 ? (int)(u / M) : (int)(u / nc);
     const int64_t o = dir == 0 ? cidx(v.pstride, v.ld, p, a, c) : cidx(v.pstride, v.ld, p, c, a);
     T qL, qR;
     recon_edges<T, RECON>(F[o - 2 * st], F[o - st], F[o], F[o + st], F[o + 2 * st], qL, qR);
@@ -290,10 +293,12 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int dir, int field
   const bool mt0 = D.mt == 1;
   const T *gc = reinterpret_cast<const T *>(D.gc);
   for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const unsigned u = t / ne;
-    const int e = (int)(t % ne) + v.i0;
-    const int a = u % M;
-    const int p = u / M;
+    // consecutive threads take consecutive columns in both directions (coalesced)
+    const unsigned w = dir == 0 ? ne : M;
+    const unsigned u = t / w, r = t % w;
+    const int e = (int)(dir == 0 ? r : u % ne) + v.i0;     // edge along dir
+    const int a = dir == 0 ? (int)(u % M) : (int)r;        // the other index
+    const int p = dir == 0 ? (int)(u / M) : (int)(u / ne);
     T uav, usg, ge_l, ge_r, gce, ge;
     int64_t cl, cr, fo;
     const T *ge_arr;
