@@ -5,11 +5,14 @@
 //                                     used by csadv_ghost_fill and for the wind basis
 //   k_ghost                           per-step ghost fill of the 3 layers a step reads, from
 //                                     flattened recipes, plus the MF-PR sum of the last step
-//   k_step<T,RECON,RK2,SPLIT>         the step: both directional passes of the split, PPM edges,
+//   k_step<T,RECON,RK2,SPLIT,WIND>    the step: both directional passes of the split, PPM edges,
 //                                     upwind fluxes, update and the mass-fixer partial sums of one
 //                                     (solve, panel, tracer, column strip, row segment) per
 //                                     wavefront (src/discrete_operators.py:18-101, src/flux.py,
-//                                     src/reconstruction_1d.py)
+//                                     src/reconstruction_1d.py); k_step1 is the same for a handle
+//                                     of one solve, whose descriptor is a kernel argument
+//   gp::k_gp_* (csadv_general.hip)    the general step (ET-S72, ET-PL07, MF-AF): one kernel per
+//                                     stage of src/discrete_operators.py:18-101
 //   k_wind_e2c / k_wind_c2e           duo-grid vector ghost fill (src/interpolation.py:347-532)
 //   k_wind_avg                        time-averaged winds (src/averaged_velocity.py:14-62)
 //   k_settle / k_finalize / k_clear_corr, k_mass / k_norms   state read-out and diagnostics
