@@ -1422,20 +1422,35 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
     const int N = d.N, M = d.N + 2 * NG;
     const int64_t edges = (int64_t)6 * M * (N + 1), cells = (int64_t)6 * M * M;
     for (int tr = 0; tr < d.ntr; ++tr) {
+      // ET-PL07 rewrites edge values between the reconstruction and the flux; otherwise the
+      // flux kernel reconstructs the upwind cell's edges itself (no edge-value arrays)
+      const bool fuse = d.et != 2;
+      auto flux = [&](int dir, int field) {
+        if (fuse)
+          hipLaunchKernelGGL((gp::k_gp_flux<T, R, true>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, dir,
+                             field);
+        else
+          hipLaunchKernelGGL((gp::k_gp_flux<T, R, false>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, dir,
+                             field);
+      };
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 0);
-      gp_recon<T, R>(h, st, s, tr, 0, 0);
-      gp_recon<T, R>(h, st, s, tr, 1, 0);
-      if (d.et == 2) gp_edges<T, R>(h, st, s, tr, 0, 0);
-      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 0, 0);
-      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 1, 0);
+      if (!fuse) {
+        gp_recon<T, R>(h, st, s, tr, 0, 0);
+        gp_recon<T, R>(h, st, s, tr, 1, 0);
+        gp_edges<T, R>(h, st, s, tr, 0, 0);
+      }
+      flux(0, 0);
+      flux(1, 0);
       hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
       // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
-      gp_recon<T, R>(h, st, s, tr, 0, 2);
-      gp_recon<T, R>(h, st, s, tr, 1, 1);
-      if (d.et == 2) gp_edges<T, R>(h, st, s, tr, 2, 1);
-      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 0, 2);
-      hipLaunchKernelGGL((gp::k_gp_flux<T>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, 1, 1);
+      if (!fuse) {
+        gp_recon<T, R>(h, st, s, tr, 0, 2);
+        gp_recon<T, R>(h, st, s, tr, 1, 1);
+        gp_edges<T, R>(h, st, s, tr, 2, 1);
+      }
+      flux(0, 2);
+      flux(1, 1);
       if (d.mf == 2) {
         const int64_t sides = (int64_t)6 * 4 * N;
         hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 0);

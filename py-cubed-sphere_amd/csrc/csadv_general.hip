@@ -283,7 +283,10 @@ __global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
 // ---------------------------------------------------------------- fluxes
 // Upwind PPM flux at x-edges i0..iend of every row (dir 0) or y-edges j0..jend of every
 // column (dir 1) from the reconstruction of field F (src/flux.py:20-128).
-template <typename T>
+// FUSE (no edge treatment between the reconstruction and the flux: ET-S72, ET-DG): the upwind
+// cell's edge values are reconstructed here from F (the same recon_edges as k_gp_recon),
+// instead of read from the edge-value arrays.
+template <typename T, int RECON, bool FUSE>
 __global__ void k_gp_flux(const Solve *solves, int s, int tr, int dir, int field) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
@@ -330,8 +333,15 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int dir, int field
     const bool up = usg >= T(0);
     const int64_t c = up ? cl : cr;
     const int ec = up ? e - 1 : e;             // upwind cell index along dir
-    const T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
-    T qL = L[c], qR = R[c], q = F[c];
+    T qL, qR, q = F[c];
+    if constexpr (FUSE) {
+      const int64_t st = dir == 0 ? 1 : v.ld;
+      recon_edges<T, RECON>(F[c - 2 * st], F[c - st], q, F[c + st], F[c + 2 * st], qL, qR);
+    } else {
+      const T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
+      qL = L[c];
+      qR = R[c];
+    }
     ge_l = ge_arr[ge_base + (int64_t)ec * ge_st];
     ge_r = ge_arr[ge_base + (int64_t)(ec + 1) * ge_st];
     gce = dir == 0 ? gc[(int64_t)a * v.ld + ec] : gc[(int64_t)ec * v.ld + a];
