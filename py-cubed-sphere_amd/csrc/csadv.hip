@@ -1378,11 +1378,13 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
 
 namespace {
 
+// x and y reconstructions in one launch, gp_blocks(total) blocks per direction
 template <typename T, int R>
-void gp_recon(csadv_t *h, hipStream_t st, int s, int tr, int dir, int field) {
+void gp_recon(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
   const csadv_solve_desc &d = h->desc[s];
   const int64_t total = (int64_t)6 * (d.N + 2 * NG) * (d.N + 2);
-  hipLaunchKernelGGL((gp::k_gp_recon<T, R>), dim3(gp_blocks(total)), dim3(BLOCK), 0, st, h->dev, s, tr, dir, field);
+  const int nbx = gp_blocks(total);
+  hipLaunchKernelGGL((gp::k_gp_recon<T, R>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field, fy_field, nbx);
 }
 
 template <typename T, int R>
@@ -1425,39 +1427,41 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
       // ET-PL07 rewrites edge values between the reconstruction and the flux; otherwise the
       // flux kernel reconstructs the upwind cell's edges itself (no edge-value arrays)
       const bool fuse = d.et != 2;
-      auto flux = [&](int dir, int field) {
+      // x- and y-edge fluxes in one launch: gp_blocks(edges) blocks per direction
+      auto flux = [&](int fx_field, int fy_field) {
+        const int nbx = gp_blocks(edges);
         if (fuse)
-          hipLaunchKernelGGL((gp::k_gp_flux<T, R, true>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, dir,
-                             field);
+          hipLaunchKernelGGL((gp::k_gp_flux<T, R, true>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
+                             fy_field, nbx);
         else
-          hipLaunchKernelGGL((gp::k_gp_flux<T, R, false>), dim3(gp_blocks(edges)), dim3(BLOCK), 0, st, h->dev, s, tr, dir,
-                             field);
+          hipLaunchKernelGGL((gp::k_gp_flux<T, R, false>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
+                             fy_field, nbx);
       };
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 0);
       if (!fuse) {
         gp_recon<T, R>(h, st, s, tr, 0, 0);
-        gp_recon<T, R>(h, st, s, tr, 1, 0);
         gp_edges<T, R>(h, st, s, tr, 0, 0);
       }
       flux(0, 0);
-      flux(1, 0);
       hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
       // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
       if (!fuse) {
-        gp_recon<T, R>(h, st, s, tr, 0, 2);
-        gp_recon<T, R>(h, st, s, tr, 1, 1);
+        gp_recon<T, R>(h, st, s, tr, 2, 1);
         gp_edges<T, R>(h, st, s, tr, 2, 1);
       }
-      flux(0, 2);
-      flux(1, 1);
+      flux(2, 1);
       if (d.mf == 2) {
         const int64_t sides = (int64_t)6 * 4 * N;
         hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 0);
         hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 1);
       }
-      hipLaunchKernelGGL((gp::k_gp_div<T>), dim3(gp::kRedGen), dim3(BLOCK), 0, st, h->dev, s, tr);
-      hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      if (d.mf == 3) {           // MF-PR: the global sum of div g sits between div and the update
+        hipLaunchKernelGGL((gp::k_gp_div<T>), dim3(gp::kRedGen), dim3(BLOCK), 0, st, h->dev, s, tr);
+        hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      } else {
+        hipLaunchKernelGGL((gp::k_gp_div_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      }
     }
     hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);
   }

@@ -156,18 +156,22 @@ __global__ void k_gp_copy_scatter(const Solve *solves, int s, int tr, int mode, 
 }
 
 // ---------------------------------------------------------------- reconstruction
-// Edge values of cells i0-1 .. iend along dir (0 = x: every row; 1 = y: every column) of
-// field F (0 = Q, 1 = Qx, 2 = Qy); src/reconstruction_1d.py:26-384.
+// Edge values of cells i0-1 .. iend along x (every row, of field fx_field) and along y (every
+// column, of field fy_field); fields 0 = Q, 1 = Qx, 2 = Qy; src/reconstruction_1d.py:26-384.
+// One launch does both directions: blocks [0, nbx) take x, the rest y (separate outputs).
 template <typename T, int RECON>
-__global__ void k_gp_recon(const Solve *solves, int s, int tr, int dir, int field) {
+__global__ void k_gp_recon(const Solve *solves, int s, int tr, int fx_field, int fy_field, int nbx) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
+  const int dir = (int)blockIdx.x >= nbx;
+  const int field = dir == 0 ? fx_field : fy_field;
+  const unsigned b = dir == 0 ? blockIdx.x : blockIdx.x - nbx, nb = dir == 0 ? nbx : gridDim.x - nbx;
   const T *F = field == 0 ? v.qin : (field == 1 ? v.qx : v.qy);
   T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
   const unsigned M = v.M, nc = v.N + 2, total = 6u * M * nc;
   const int st = dir == 0 ? 1 : v.ld;
-  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (unsigned t = b * blockDim.x + threadIdx.x; t < total; t += nb * blockDim.x) {
     // consecutive threads take consecutive columns in both directions (coalesced): along the
     // row in x, across the columns in y
     const unsigned w = dir == 0 ? nc : M;
@@ -281,21 +285,26 @@ __global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
 }
 
 // ---------------------------------------------------------------- fluxes
-// Upwind PPM flux at x-edges i0..iend of every row (dir 0) or y-edges j0..jend of every
-// column (dir 1) from the reconstruction of field F (src/flux.py:20-128).
+// Upwind PPM fluxes at x-edges i0..iend of every row (dir 0, from field fx_field) and at
+// y-edges j0..jend of every column (dir 1, from field fy_field) from the reconstruction of
+// the field (src/flux.py:20-128). One launch does both directions: blocks [0, nbx) take the
+// x-edges, the rest the y-edges; the two read their own inputs and write separate arrays.
 // FUSE (no edge treatment between the reconstruction and the flux: ET-S72, ET-DG): the upwind
 // cell's edge values are reconstructed here from F (the same recon_edges as k_gp_recon),
 // instead of read from the edge-value arrays.
 template <typename T, int RECON, bool FUSE>
-__global__ void k_gp_flux(const Solve *solves, int s, int tr, int dir, int field) {
+__global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int fy_field, int nbx) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
+  const int dir = (int)blockIdx.x >= nbx;
+  const int field = dir == 0 ? fx_field : fy_field;
+  const unsigned b = dir == 0 ? blockIdx.x : blockIdx.x - nbx, nb = dir == 0 ? nbx : gridDim.x - nbx;
   const T *F = field == 0 ? v.qin : (field == 1 ? v.qx : v.qy);
   const unsigned M = v.M, ne = v.N + 1, total = 6u * M * ne;
   const bool mt0 = D.mt == 1;
   const T *gc = reinterpret_cast<const T *>(D.gc);
-  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (unsigned t = b * blockDim.x + threadIdx.x; t < total; t += nb * blockDim.x) {
     // consecutive threads take consecutive columns in both directions (coalesced)
     const unsigned w = dir == 0 ? ne : M;
     const unsigned u = t / w, r = t % w;
@@ -511,6 +520,32 @@ __global__ void k_gp_update(const Solve *solves, int s, int tr) {
     T dv = v.div[c];                       // every cell of div exists; only interior ones are used
     if (pr) dv = dv - (gc[(int64_t)j * v.ld + i] * T(m0)) / T(D.a2);
     v.qout[c] = interior ? q - T(D.dt) * dv : q;
+  }
+}
+
+// Without MF-PR no reduction separates the divergence from the update: one pass forms div of
+// each interior cell in a register and writes Q^{n+1} = Q^n - dt div (the same operations as
+// k_gp_div then k_gp_update); every other cell is copied.
+template <typename T>
+__global__ void k_gp_div_update(const Solve *solves, int s, int tr) {
+  const View<T> v = view<T>(solves, s, tr);
+  const csadv_solve_desc &D = v.S->d;
+  if (finished(D)) return;
+  const unsigned M = v.M, total = 6u * M * M;
+  const T *gc = reinterpret_cast<const T *>(D.gc);
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / M;
+    const int i = t % M, j = u % M, p = u / M;
+    const int64_t c = cidx(v.pstride, v.ld, p, j, i);
+    const bool interior = i >= v.i0 && i < v.iend && j >= v.i0 && j < v.iend;
+    const T q = v.qin[c];
+    if (interior) {
+      const T g = gc[(int64_t)j * v.ld + i];
+      const T dv = -(dflux_x(v, D, p, j, i) + dflux_y(v, D, p, j, i)) / (T(D.dt) * g);
+      v.qout[c] = q - T(D.dt) * dv;
+    } else {
+      v.qout[c] = q;
+    }
   }
 }
 
