@@ -1451,16 +1451,18 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
         gp_edges<T, R>(h, st, s, tr, 2, 1);
       }
       flux(2, 1);
-      if (d.mf == 2) {
+      if (d.mf == 2) {           // MF-AF: gather the cube-edge averages; k_gp_div_update reads them
         const int64_t sides = (int64_t)6 * 4 * N;
-        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 0);
-        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 1);
+        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
       }
       if (d.mf == 3) {           // MF-PR: the global sum of div g sits between div and the update
         hipLaunchKernelGGL((gp::k_gp_div<T>), dim3(gp::kRedGen), dim3(BLOCK), 0, st, h->dev, s, tr);
         hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
       } else {
-        hipLaunchKernelGGL((gp::k_gp_div_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+        if (d.mf == 2)
+          hipLaunchKernelGGL((gp::k_gp_div_update<T, true>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+        else
+          hipLaunchKernelGGL((gp::k_gp_div_update<T, false>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
       }
     }
     hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);

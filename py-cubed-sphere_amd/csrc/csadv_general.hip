@@ -368,9 +368,10 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int 
 }
 
 // MF-AF (src/edges_treatment.py:231-278): each cube-edge flux becomes 0.5 f + 0.5 (sign f'),
-// f' the neighbour's flux on the same edge; gather all 24 sides, then write.
+// f' the neighbour's flux on the same edge. All 24 sides are gathered into avg (entry
+// (p * 4 + side) * N + k - i0); k_gp_div_update reads them there in place of the fluxes.
 template <typename T>
-__global__ void k_gp_flux_avg(const Solve *solves, int s, int tr, int write) {
+__global__ void k_gp_flux_avg(const Solve *solves, int s, int tr) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
@@ -387,11 +388,7 @@ __global__ void k_gp_flux_avg(const Solve *solves, int s, int tr, int write) {
     const unsigned u = t / N;
     const int k = (int)(t % N) + v.i0;
     const int side = u & 3, p = u >> 2;
-    T *own = fidx(p, side, k);
-    if (write) {
-      *own = v.avg[t];
-      continue;
-    }
+    const T *own = fidx(p, side, k);
     const int32_t *o = D.orient + (p * 4 + side) * 7;
     int q, i, j, sq, sign;
     halo_src(o, 0, k, q, i, j);
@@ -405,57 +402,68 @@ __global__ void k_gp_flux_avg(const Solve *solves, int s, int tr, int write) {
 // dF of cell i (x) / j (y) from the fluxes: -(f[c+1] - f[c]) * dt / dx inside [i0, iend), else 0
 // (the edges read exist for every cell of the panel: i + 1 <= M < ldu, j + 1 <= M rows of M + 1)
 template <typename T>
-__device__ __forceinline__ T dflux_x(const View<T> &v, const csadv_solve_desc &D, int p, int j, int i) {
-  const T *f = v.fx + p * v.pstride_u + (int64_t)j * v.ldu;
-  const T d = -(f[i + 1] - f[i]);
-  const T r = d * T(D.dt) / T(D.dx);
+__device__ __forceinline__ T dflux(T f1, T f0, T dt, T dx) {
+  const T d = -(f1 - f0);
+  return d * dt / dx;
+}
+template <typename T>
+__device__ __forceinline__ T dflux_x(const View<T> &v, const T *fx, const csadv_solve_desc &D, int p, int j, int i) {
+  const T *f = fx + p * v.pstride_u + (int64_t)j * v.ldu;
+  const T r = dflux(f[i + 1], f[i], T(D.dt), T(D.dx));
   return (i >= v.i0 && i < v.iend) ? r : T(0);
 }
 template <typename T>
-__device__ __forceinline__ T dflux_y(const View<T> &v, const csadv_solve_desc &D, int p, int j, int i) {
-  const T *f = v.fy + p * v.pstride_v + i;
-  const T d = -(f[(int64_t)(j + 1) * v.ld] - f[(int64_t)j * v.ld]);
-  const T r = d * T(D.dt) / T(D.dy);
+__device__ __forceinline__ T dflux_y(const View<T> &v, const T *fy, const csadv_solve_desc &D, int p, int j, int i) {
+  const T *f = fy + p * v.pstride_v + i;
+  const T r = dflux(f[(int64_t)(j + 1) * v.ld], f[(int64_t)j * v.ld], T(D.dt), T(D.dy));
   return (j >= v.i0 && j < v.iend) ? r : T(0);
 }
 
-// Qx, Qy of every cell (src/discrete_operators.py:31-73)
+// Qx (WHICH 0) or Qy (WHICH 1) of cell (p, j, i) from Q and the first pass's fluxes
+// (src/discrete_operators.py:31-73).
+template <typename T, int WHICH>
+__device__ __forceinline__ T qsplit(const View<T> &v, const csadv_solve_desc &D, int p, int j, int i) {
+  const T *gc = reinterpret_cast<const T *>(D.gc);
+  const T Q = v.qin[cidx(v.pstride, v.ld, p, j, i)], g = gc[(int64_t)j * v.ld + i];
+  const T dF = WHICH == 0 ? dflux_x(v, v.fx, D, p, j, i) : dflux_y(v, v.fy, D, p, j, i);
+  if (D.split == 1) {
+    const T gQ = Q * g;
+    return (gQ + T(0.5) * dF) / g;
+  }
+  const T dt = T(D.dt);                    // cx = u dt / dx (src/cfl.py:10-19)
+  T c1, c2;
+  if (WHICH == 0) {
+    const T *U = reinterpret_cast<const T *>(D.uavg) + p * v.pstride_u, *gu = reinterpret_cast<const T *>(D.gu);
+    const T dx = T(D.dx);
+    const T cx1 = (U[(int64_t)j * v.ldu + i + 1] * dt) / dx, cx2 = (U[(int64_t)j * v.ldu + i] * dt) / dx;
+    c1 = gu[(int64_t)j * v.ldu + i + 1] * cx1;
+    c2 = gu[(int64_t)j * v.ldu + i] * cx2;
+  } else {
+    const T *V = reinterpret_cast<const T *>(D.vavg) + p * v.pstride_v, *gv = reinterpret_cast<const T *>(D.gv);
+    const T dy = T(D.dy);
+    const T cy1 = (V[(int64_t)(j + 1) * v.ld + i] * dt) / dy, cy2 = (V[(int64_t)j * v.ld + i] * dt) / dy;
+    c1 = gv[(int64_t)(j + 1) * v.ld + i] * cy1;
+    c2 = gv[(int64_t)j * v.ld + i] * cy2;
+  }
+  if (D.split == 2) {
+    const T gQ = Q * g;
+    return (gQ + T(0.5) * dF + T(0.5) * (c1 - c2) * Q) / g;
+  }
+  return T(0.5) * (Q + (Q + dF) / (T(1.0) - (c1 - c2)));
+}
+
+// Qx, Qy of every cell
 template <typename T>
 __global__ void k_gp_split(const Solve *solves, int s, int tr) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
   const unsigned M = v.M, total = 6u * M * M;
-  const T *gc = reinterpret_cast<const T *>(D.gc), *gu = reinterpret_cast<const T *>(D.gu),
-          *gv = reinterpret_cast<const T *>(D.gv);
   for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const unsigned u = t / M;
     const int i = t % M, j = u % M, p = u / M;
     const int64_t c = cidx(v.pstride, v.ld, p, j, i);
-    const T Q = v.qin[c], g = gc[(int64_t)j * v.ld + i];
-    const T dFx = dflux_x(v, D, p, j, i), dFy = dflux_y(v, D, p, j, i);
-    T Qx, Qy;
-    if (D.split == 1) {
-      const T gQ = Q * g;
-      Qx = (gQ + T(0.5) * dFx) / g;
-      Qy = (gQ + T(0.5) * dFy) / g;
-    } else {
-      const T *U = reinterpret_cast<const T *>(D.uavg) + p * v.pstride_u;
-      const T *V = reinterpret_cast<const T *>(D.vavg) + p * v.pstride_v;
-      const T dt = T(D.dt), dx = T(D.dx), dy = T(D.dy);     // cx = u dt / dx (src/cfl.py:10-19)
-      const T cx1 = (U[(int64_t)j * v.ldu + i + 1] * dt) / dx, cx2 = (U[(int64_t)j * v.ldu + i] * dt) / dx;
-      const T cy1 = (V[(int64_t)(j + 1) * v.ld + i] * dt) / dy, cy2 = (V[(int64_t)j * v.ld + i] * dt) / dy;
-      const T c1x = gu[(int64_t)j * v.ldu + i + 1] * cx1, c2x = gu[(int64_t)j * v.ldu + i] * cx2;
-      const T c1y = gv[(int64_t)(j + 1) * v.ld + i] * cy1, c2y = gv[(int64_t)j * v.ld + i] * cy2;
-      if (D.split == 2) {
-        const T gQ = Q * g;
-        Qx = (gQ + T(0.5) * dFx + T(0.5) * (c1x - c2x) * Q) / g;
-        Qy = (gQ + T(0.5) * dFy + T(0.5) * (c1y - c2y) * Q) / g;
-      } else {
-        Qx = T(0.5) * (Q + (Q + dFx) / (T(1.0) - (c1x - c2x)));
-        Qy = T(0.5) * (Q + (Q + dFy) / (T(1.0) - (c1y - c2y)));
-      }
-    }
+    const T Qx = qsplit<T, 0>(v, D, p, j, i), Qy = qsplit<T, 1>(v, D, p, j, i);
     v.qx[c] = Qx;
     v.qy[c] = Qy;
   }
@@ -475,7 +483,7 @@ __global__ void k_gp_div(const Solve *solves, int s, int tr) {
     const unsigned u = t / N;
     const int i = (int)(t % N) + v.i0, j = (int)(u % N) + v.i0, p = u / N;
     const T g = gc[(int64_t)j * v.ld + i];
-    const T dv = -(dflux_x(v, D, p, j, i) + dflux_y(v, D, p, j, i)) / (T(D.dt) * g);
+    const T dv = -(dflux_x(v, v.fx, D, p, j, i) + dflux_y(v, v.fy, D, p, j, i)) / (T(D.dt) * g);
     v.div[cidx(v.pstride, v.ld, p, j, i)] = dv;
     acc += (double)dv * (double)g;
   }
@@ -525,8 +533,10 @@ __global__ void k_gp_update(const Solve *solves, int s, int tr) {
 
 // Without MF-PR no reduction separates the divergence from the update: one pass forms div of
 // each interior cell in a register and writes Q^{n+1} = Q^n - dt div (the same operations as
-// k_gp_div then k_gp_update); every other cell is copied.
-template <typename T>
+// k_gp_div then k_gp_update); every other cell is copied. AVG (MF-AF): the fluxes on the
+// panel sides are the cube-edge averages that k_gp_flux_avg gathered (side order E, W, N, S,
+// N entries per side), read in place of the flux arrays' entries there.
+template <typename T, bool AVG>
 __global__ void k_gp_div_update(const Solve *solves, int s, int tr) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
@@ -540,8 +550,18 @@ __global__ void k_gp_div_update(const Solve *solves, int s, int tr) {
     const bool interior = i >= v.i0 && i < v.iend && j >= v.i0 && j < v.iend;
     const T q = v.qin[c];
     if (interior) {
+      const T *fxr = v.fx + p * v.pstride_u + (int64_t)j * v.ldu, *fyc = v.fy + p * v.pstride_v + i;
+      T fw = fxr[i], fe = fxr[i + 1], fs = fyc[(int64_t)j * v.ld], fn = fyc[(int64_t)(j + 1) * v.ld];
+      if constexpr (AVG) {
+        const T *side = v.avg + (int64_t)p * 4 * v.N;
+        if (i + 1 == v.iend) fe = side[0 * v.N + j - v.i0];
+        if (i == v.i0) fw = side[1 * v.N + j - v.i0];
+        if (j + 1 == v.iend) fn = side[2 * v.N + i - v.i0];
+        if (j == v.i0) fs = side[3 * v.N + i - v.i0];
+      }
       const T g = gc[(int64_t)j * v.ld + i];
-      const T dv = -(dflux_x(v, D, p, j, i) + dflux_y(v, D, p, j, i)) / (T(D.dt) * g);
+      const T dFx = dflux(fe, fw, T(D.dt), T(D.dx)), dFy = dflux(fn, fs, T(D.dt), T(D.dy));
+      const T dv = -(dFx + dFy) / (T(D.dt) * g);
       v.qout[c] = q - T(D.dt) * dv;
     } else {
       v.qout[c] = q;

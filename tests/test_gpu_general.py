@@ -38,6 +38,8 @@ GENERAL_OPTS = [
     dict(recon=1, opsplit=2, et=2, mt=1, dp=2, mf=2),        # ET-PL07 + MF-AF, PPM-0, L04 split
     dict(recon=2, opsplit=1, et=2, mt=1, dp=1, mf=3),        # CW84 with ET-PL07
     dict(recon=4, opsplit=3, et=1, mt=2, dp=1, mf=2),        # L04 recon, ET-S72, MF-AF
+    dict(recon=2, opsplit=3, et=3, mt=2, dp=1, mf=2),        # ET-DG + MF-AF, CW84, PL07 split
+    dict(recon=1, opsplit=2, et=3, mt=1, dp=2, mf=2),        # ET-DG + MF-AF, PPM-0, L04 split, RK2
 ]
 
 
