@@ -1393,8 +1393,7 @@ void gp_edges(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_fi
   const int64_t sides = (int64_t)6 * 4 * d.N;
   hipLaunchKernelGGL((gp::k_gp_extrap<T, R>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
                      fy_field);
-  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 0);
-  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, 1);
+  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
   hipLaunchKernelGGL((gp::k_gp_edge_ghosts<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
 }
 
@@ -1403,10 +1402,8 @@ void gp_copy_halo(csadv_t *h, hipStream_t st, int s, int tr, int mode) {
   const int M = h->desc[s].N + 2 * NG;
   hipLaunchKernelGGL((gp::k_gp_copy_gather<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
                      mode);
-  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)48 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
-                     mode, 0);
-  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)48 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
-                     mode, 1);
+  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
+                     mode);
 }
 
 // One step of every solve by the general step (src/discrete_operators.py:18-101 stage by stage).

@@ -135,23 +135,24 @@ __global__ void k_gp_copy_gather(const Solve *solves, int s, int tr, int mode) {
   }
 }
 
-// Scatter of one side group (E/W, or N/S) into its target field: E/W ghosts go to Qy (or Q),
-// N/S ghosts to Qx (or Q).  The N/S group is launched after the E/W group, so where the
-// two overlap in one field (mode 0) the N/S values are the ones that stay, as in the
-// reference's order.
+// Scatter of the gathered strips into their target fields: E/W ghosts go to Qy (or Q), N/S
+// ghosts to Qx (or Q). In mode 0 both land in Q and overlap in the corner blocks, where the
+// reference's order (E/W first, then N/S) leaves the N/S values: E/W lanes skip those rows.
+// Qx and Qy are consecutive in the work buffer, so the field is an offset from Qx.
 template <typename T>
-__global__ void k_gp_copy_scatter(const Solve *solves, int s, int tr, int mode, int ns_group) {
+__global__ void k_gp_copy_scatter(const Solve *solves, int s, int tr, int mode) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
-  const unsigned M = v.M, total = 48u * M;
-  T *dst = mode == 0 ? v.qin : (ns_group ? v.qx : v.qy);
+  const unsigned M = v.M, total = 96u * M;
   for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const unsigned r = t % M, u = t / M;              // u = (p * 2 + side - group) * 4 + d
-    const int d = u & 3, side = ((u >> 2) & 1) + (ns_group ? 2 : 0), p = u >> 3;
+    const unsigned r = t % M, u = t / M;              // u = (p * 4 + side) * 4 + d, as gathered
+    const int d = u & 3, side = (u >> 2) & 3, p = u >> 4;
+    if (mode == 0 && side < 2 && (r < (unsigned)NG || r >= (unsigned)(v.N + NG))) continue;
     int ti, tj;
     ghost_target(side, d, r, v.N, ti, tj);
-    dst[cidx(v.pstride, v.ld, p, tj, ti)] = v.strip[((p * 4 + side) * 4 + d) * M + r];
+    T *dst = mode == 0 ? v.qin : v.qx + (side < 2 ? v.cell : 0);
+    dst[cidx(v.pstride, v.ld, p, tj, ti)] = v.strip[t];
   }
 }
 
@@ -229,10 +230,11 @@ __device__ __forceinline__ int64_t boundary_cell(const View<T> &v, int p, int si
   return xdir ? cidx(v.pstride, v.ld, p, k, b) : cidx(v.pstride, v.ld, p, b, k);
 }
 
-// Cube-edge averaging of the facing edge values (src/edges_treatment.py:31-76): gather the
-// averages of all 24 sides, then write them (two launches: the values read are written).
+// Cube-edge averaging of the facing edge values (src/edges_treatment.py:31-76): the averages
+// of all 24 sides are gathered into avg (entry (p * 4 + side) * N + k - i0); k_gp_edge_ghosts
+// writes them back.
 template <typename T>
-__global__ void k_gp_edge_avg(const Solve *solves, int s, int tr, int write) {
+__global__ void k_gp_edge_avg(const Solve *solves, int s, int tr) {
   const View<T> v = view<T>(solves, s, tr);
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
@@ -241,12 +243,8 @@ __global__ void k_gp_edge_avg(const Solve *solves, int s, int tr, int write) {
     const unsigned u = t / N;
     const int k = (int)(t % N) + v.i0;
     const int side = u & 3, p = u >> 2;
-    T *own = parab(v, side ^ 1);
+    const T *own = parab(v, side ^ 1);
     const int64_t oi = boundary_cell(v, p, side, k);
-    if (write) {
-      own[oi] = v.avg[t];
-      continue;
-    }
     const int32_t *o = D.orient + (p * 4 + side) * 7;
     int q, i, j, sq, sign;
     halo_src(o, 0, k, q, i, j);
@@ -256,8 +254,12 @@ __global__ void k_gp_edge_avg(const Solve *solves, int s, int tr, int write) {
   }
 }
 
-// Ghost-cell parabolas (src/edges_treatment.py:119-178): the ghost cell next to (p, side)
-// takes the neighbour's boundary-cell edge values, swapped when the axes point opposite ways.
+// The averages written back to the boundary cells' facing edges, and the ghost-cell
+// parabolas (src/edges_treatment.py:119-178): the ghost cell next to (p, side) takes the
+// neighbour's boundary-cell edge values, swapped when the axes point opposite ways. The
+// neighbour's facing value is its average, read from avg; its other value (slot sq) is one
+// that no lane here writes: a boundary cell's facing slots are written only for its own
+// sides, and with N >= 8 no cell lies on two opposite sides.
 template <typename T>
 __global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
   const View<T> v = view<T>(solves, s, tr);
@@ -268,13 +270,16 @@ __global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
     const unsigned u = t / N;
     const int k = (int)(t % N) + v.i0;
     const int side = u & 3, p = u >> 2;
+    parab(v, side ^ 1)[boundary_cell(v, p, side, k)] = v.avg[t];
     const int32_t *o = D.orient + (p * 4 + side) * 7;
     int q, i, j, sq, sign;
     halo_src(o, 0, k, q, i, j);
     across(o, side, sq, sign);
-    const int nslot = sq < 2 ? 0 : 2;                // the neighbour's x or y edge values
     const int64_t ni = cidx(v.pstride, v.ld, q, j, i);
-    const T l = parab(v, nslot)[ni], r = parab(v, nslot + 1)[ni];
+    const T other = parab(v, sq)[ni];
+    const T facing = v.avg[(q * 4 + sq) * (int)N + (sq < 2 ? j : i) - v.i0];
+    const bool facing_r = sq == 0 || sq == 2;        // E, N face with q_R; W, S with q_L
+    const T l = facing_r ? other : facing, r = facing_r ? facing : other;
     const bool xdir = side < 2;
     const int g = (side == 0 || side == 2) ? v.iend : v.i0 - 1;
     const int64_t gi = xdir ? cidx(v.pstride, v.ld, p, k, g) : cidx(v.pstride, v.ld, p, g, k);
