@@ -1378,11 +1378,12 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
 
 namespace {
 
-// x and y reconstructions in one launch, gp_blocks(total) blocks per direction
+// x and y reconstructions (of the cells the ET-PL07 edge treatment uses) in one launch,
+// gp_blocks(total) blocks per direction
 template <typename T, int R>
 void gp_recon(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
   const csadv_solve_desc &d = h->desc[s];
-  const int64_t total = (int64_t)6 * (d.N + 2 * NG) * (d.N + 2);
+  const int64_t total = (int64_t)36 * d.N;
   const int nbx = gp_blocks(total);
   hipLaunchKernelGGL((gp::k_gp_recon<T, R>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field, fy_field, nbx);
 }
@@ -1421,8 +1422,9 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
     const int N = d.N, M = d.N + 2 * NG;
     const int64_t edges = (int64_t)6 * M * (N + 1), cells = (int64_t)6 * M * M;
     for (int tr = 0; tr < d.ntr; ++tr) {
-      // ET-PL07 rewrites edge values between the reconstruction and the flux; otherwise the
-      // flux kernel reconstructs the upwind cell's edges itself (no edge-value arrays)
+      // ET-PL07 rewrites edge values next to the panel sides between the reconstruction and
+      // the flux, which reads those from the arrays; every other edge value the flux kernel
+      // reconstructs itself
       const bool fuse = d.et != 2;
       // x- and y-edge fluxes in one launch: gp_blocks(edges) blocks per direction
       auto flux = [&](int fx_field, int fy_field) {

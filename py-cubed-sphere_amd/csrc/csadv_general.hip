@@ -157,9 +157,11 @@ __global__ void k_gp_copy_scatter(const Solve *solves, int s, int tr, int mode) 
 }
 
 // ---------------------------------------------------------------- reconstruction
-// Edge values of cells i0-1 .. iend along x (every row, of field fx_field) and along y (every
-// column, of field fy_field); fields 0 = Q, 1 = Qx, 2 = Qy; src/reconstruction_1d.py:26-384.
-// One launch does both directions: blocks [0, nbx) take x, the rest y (separate outputs).
+// Edge values (src/reconstruction_1d.py:26-384) of the cells that the ET-PL07 edge treatment
+// reads or rewrites: cells i0 .. i0+2 and iend-3 .. iend-1 along x (rows i0 .. iend-1, of
+// field fx_field) and along y (columns i0 .. iend-1, of field fy_field); fields 0 = Q,
+// 1 = Qx, 2 = Qy. The flux reconstructs every other cell itself. One launch does both
+// directions: blocks [0, nbx) take x, the rest y (separate outputs).
 template <typename T, int RECON>
 __global__ void k_gp_recon(const Solve *solves, int s, int tr, int fx_field, int fy_field, int nbx) {
   const View<T> v = view<T>(solves, s, tr);
@@ -170,16 +172,13 @@ __global__ void k_gp_recon(const Solve *solves, int s, int tr, int fx_field, int
   const unsigned b = dir == 0 ? blockIdx.x : blockIdx.x - nbx, nb = dir == 0 ? nbx : gridDim.x - nbx;
   const T *F = field == 0 ? v.qin : (field == 1 ? v.qx : v.qy);
   T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
-  const unsigned M = v.M, nc = v.N + 2, total = 6u * M * nc;
+  const unsigned N = v.N, total = 36u * N;
   const int st = dir == 0 ? 1 : v.ld;
   for (unsigned t = b * blockDim.x + threadIdx.x; t < total; t += nb * blockDim.x) {
-    // consecutive threads take consecutive columns in both directions (coalesced): along the
-    // row in x, across the columns in y
-    const unsigned w = dir == 0 ? nc : M;
-    const unsigned u = t / w, r = t % w;
-    const int c = (int)(dir == 0 ? r : u % nc) + v.i0 - 1;   // cell along dir
-    const int a = dir == 0 ? (int)(u % M) : (int)r;          // the other index
-    const int p = dir == 0 ? (int)(u / M) : (int)(u / nc);
+    const unsigned c6 = t % 6, u = t / 6;
+    const int c = c6 < 3 ? v.i0 + (int)c6 : v.iend - 6 + (int)c6;   // cell along dir
+    const int a = (int)(u % N) + v.i0;                                // the other index
+    const int p = (int)(u / N);
     const int64_t o = dir == 0 ? cidx(v.pstride, v.ld, p, a, c) : cidx(v.pstride, v.ld, p, c, a);
     T qL, qR;
     recon_edges<T, RECON>(F[o - 2 * st], F[o - st], F[o], F[o + st], F[o + 2 * st], qL, qR);
@@ -294,9 +293,10 @@ __global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
 // y-edges j0..jend of every column (dir 1, from field fy_field) from the reconstruction of
 // the field (src/flux.py:20-128). One launch does both directions: blocks [0, nbx) take the
 // x-edges, the rest the y-edges; the two read their own inputs and write separate arrays.
-// FUSE (no edge treatment between the reconstruction and the flux: ET-S72, ET-DG): the upwind
-// cell's edge values are reconstructed here from F (the same recon_edges as k_gp_recon),
-// instead of read from the edge-value arrays.
+// The upwind cell's edge values are reconstructed here from F (the same recon_edges as
+// k_gp_recon). Without FUSE (ET-PL07, whose edge treatment runs between the reconstruction
+// and the flux), the cells next to each panel side on the interior rows -- the three inside
+// and the ghost cell, whose edge values the treatment rewrote -- read them from the arrays.
 template <typename T, int RECON, bool FUSE>
 __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int fy_field, int nbx) {
   const View<T> v = view<T>(solves, s, tr);
@@ -348,13 +348,14 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int 
     const int64_t c = up ? cl : cr;
     const int ec = up ? e - 1 : e;             // upwind cell index along dir
     T qL, qR, q = F[c];
-    if constexpr (FUSE) {
-      const int64_t st = dir == 0 ? 1 : v.ld;
-      recon_edges<T, RECON>(F[c - 2 * st], F[c - st], q, F[c + st], F[c + 2 * st], qL, qR);
-    } else {
-      const T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
-      qL = L[c];
-      qR = R[c];
+    const int64_t st = dir == 0 ? 1 : v.ld;
+    recon_edges<T, RECON>(F[c - 2 * st], F[c - st], q, F[c + st], F[c + 2 * st], qL, qR);
+    if constexpr (!FUSE) {
+      if (a >= v.i0 && a < v.iend && (ec <= v.i0 + 1 || ec >= v.iend - 2)) {
+        const T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
+        qL = L[c];
+        qR = R[c];
+      }
     }
     ge_l = ge_arr[ge_base + (int64_t)ec * ge_st];
     ge_r = ge_arr[ge_base + (int64_t)(ec + 1) * ge_st];
