@@ -1378,16 +1378,8 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
 
 namespace {
 
-// x and y reconstructions (of the cells the ET-PL07 edge treatment uses) in one launch,
-// gp_blocks(total) blocks per direction
-template <typename T, int R>
-void gp_recon(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
-  const csadv_solve_desc &d = h->desc[s];
-  const int64_t total = (int64_t)36 * d.N;
-  const int nbx = gp_blocks(total);
-  hipLaunchKernelGGL((gp::k_gp_recon<T, R>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field, fy_field, nbx);
-}
-
+// ET-PL07 edge values: reconstruction of the cells next to the panel sides with the
+// extrapolation, cube-edge averages, their write-back with the ghost parabolas
 template <typename T, int R>
 void gp_edges(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
   const csadv_solve_desc &d = h->desc[s];
@@ -1438,7 +1430,6 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
       };
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 0);
       if (!fuse) {
-        gp_recon<T, R>(h, st, s, tr, 0, 0);
         gp_edges<T, R>(h, st, s, tr, 0, 0);
       }
       flux(0, 0);
@@ -1446,7 +1437,6 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
       // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
       if (!fuse) {
-        gp_recon<T, R>(h, st, s, tr, 2, 1);
         gp_edges<T, R>(h, st, s, tr, 2, 1);
       }
       flux(2, 1);

@@ -4,7 +4,7 @@
 // order of src/discrete_operators.py:18-101:
 //
 //   copy halo of Q (ET-S72/PL07; ET-DG uses the duo-grid fill)   src/interpolation.py:320-340
-//   x / y reconstruction of Q                                    src/reconstruction_1d.py:26-384
+//   x / y reconstruction of Q (in the flux; for ET-PL07 also beside the panel sides) src/reconstruction_1d.py:26-384
 //   [ET-PL07] edge extrapolation, cube-edge averaging, ghosts    src/edges_treatment.py:82-178
 //   x / y upwind fluxes                                          src/flux.py:20-128
 //   F / G operators and the split: Qx, Qy                        src/discrete_operators.py:31-73
@@ -156,39 +156,12 @@ __global__ void k_gp_copy_scatter(const Solve *solves, int s, int tr, int mode) 
   }
 }
 
-// ---------------------------------------------------------------- reconstruction
-// Edge values (src/reconstruction_1d.py:26-384) of the cells that the ET-PL07 edge treatment
-// reads or rewrites: cells i0 .. i0+2 and iend-3 .. iend-1 along x (rows i0 .. iend-1, of
-// field fx_field) and along y (columns i0 .. iend-1, of field fy_field); fields 0 = Q,
-// 1 = Qx, 2 = Qy. The flux reconstructs every other cell itself. One launch does both
-// directions: blocks [0, nbx) take x, the rest y (separate outputs).
-template <typename T, int RECON>
-__global__ void k_gp_recon(const Solve *solves, int s, int tr, int fx_field, int fy_field, int nbx) {
-  const View<T> v = view<T>(solves, s, tr);
-  const csadv_solve_desc &D = v.S->d;
-  if (finished(D)) return;
-  const int dir = (int)blockIdx.x >= nbx;
-  const int field = dir == 0 ? fx_field : fy_field;
-  const unsigned b = dir == 0 ? blockIdx.x : blockIdx.x - nbx, nb = dir == 0 ? nbx : gridDim.x - nbx;
-  const T *F = field == 0 ? v.qin : (field == 1 ? v.qx : v.qy);
-  T *L = dir == 0 ? v.pxl : v.pyl, *R = dir == 0 ? v.pxr : v.pyr;
-  const unsigned N = v.N, total = 36u * N;
-  const int st = dir == 0 ? 1 : v.ld;
-  for (unsigned t = b * blockDim.x + threadIdx.x; t < total; t += nb * blockDim.x) {
-    const unsigned c6 = t % 6, u = t / 6;
-    const int c = c6 < 3 ? v.i0 + (int)c6 : v.iend - 6 + (int)c6;   // cell along dir
-    const int a = (int)(u % N) + v.i0;                                // the other index
-    const int p = (int)(u / N);
-    const int64_t o = dir == 0 ? cidx(v.pstride, v.ld, p, a, c) : cidx(v.pstride, v.ld, p, c, a);
-    T qL, qR;
-    recon_edges<T, RECON>(F[o - 2 * st], F[o - st], F[o], F[o + st], F[o + 2 * st], qL, qR);
-    L[o] = qL;
-    R[o] = qR;
-  }
-}
-
-// PL07 edge extrapolation at the four panel sides (eqs. 47 and 49;
-// src/edges_treatment.py:82-114): one thread per (panel, side, interior position).
+// ---------------------------------------------------------------- ET-PL07 edge values
+// One thread per (panel, side, interior position k): the PPM edge values
+// (src/reconstruction_1d.py:26-384) of the three cells inside the side along the side's
+// axis -- the only cells whose arrays the flux reads; it reconstructs every other cell
+// itself -- then the PL07 edge extrapolation (eqs. 47 and 49; src/edges_treatment.py:82-114)
+// over them. x-sides use field fx_field, y-sides fy_field (0 = Q, 1 = Qx, 2 = Qy).
 template <typename T, int RECON>
 __global__ void k_gp_extrap(const Solve *solves, int s, int tr, int fx_field, int fy_field) {
   const View<T> v = view<T>(solves, s, tr);
@@ -210,12 +183,22 @@ __global__ void k_gp_extrap(const Solve *solves, int s, int tr, int fx_field, in
                        (int64_t)(low ? i0 : iend - 1) * st;
     // A: the edge on the panel side (q_L at the low end, q_R at the high end); B: the other
     T *A = parab(v, (xdir ? 0 : 2) + (low ? 0 : 1)), *B = parab(v, (xdir ? 0 : 2) + (low ? 1 : 0));
+    T a2 = T(0);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int64_t o = c0 + (int64_t)m * dir;
+      T qL, qR;
+      recon_edges<T, RECON>(F[o - 2 * st], F[o - st], F[o], F[o + st], F[o + 2 * st], qL, qR);
+      A[o] = low ? qL : qR;
+      B[o] = low ? qR : qL;
+      if (m == 2) a2 = low ? qL : qR;
+    }
     const T Q0 = F[c0], Q1 = F[c0 + dir], Q2 = F[c0 + 2 * dir];
     A[c0] = T(1.5) * Q0 - T(0.5) * Q1;
     const T e = (T(3.0) * Q0 + T(11.0) * Q1 - T(2.0) * (Q2 - Q0)) / T(14.0);
     B[c0] = e;
     A[c0 + dir] = e;
-    if (RECON == 3) B[c0 + dir] = A[c0 + 2 * dir];
+    if (RECON == 3) B[c0 + dir] = a2;
   }
 }
 
@@ -294,9 +277,9 @@ __global__ void k_gp_edge_ghosts(const Solve *solves, int s, int tr) {
 // the field (src/flux.py:20-128). One launch does both directions: blocks [0, nbx) take the
 // x-edges, the rest the y-edges; the two read their own inputs and write separate arrays.
 // The upwind cell's edge values are reconstructed here from F (the same recon_edges as
-// k_gp_recon). Without FUSE (ET-PL07, whose edge treatment runs between the reconstruction
+// k_gp_extrap). Without FUSE (ET-PL07, whose edge treatment runs between the reconstruction
 // and the flux), the cells next to each panel side on the interior rows -- the three inside
-// and the ghost cell, whose edge values the treatment rewrote -- read them from the arrays.
+// and the ghost cell, whose edge values the treatment wrote -- read them from the arrays.
 template <typename T, int RECON, bool FUSE>
 __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int fy_field, int nbx) {
   const View<T> v = view<T>(solves, s, tr);
