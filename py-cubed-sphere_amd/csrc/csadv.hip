@@ -1418,9 +1418,10 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
       // the flux, which reads those from the arrays; every other edge value the flux kernel
       // reconstructs itself
       const bool fuse = d.et != 2;
-      // x- and y-edge fluxes in one launch: gp_blocks(edges) blocks per direction
+      // x- and y-edge fluxes in one launch: gp_blocks(edges / 2) blocks per direction, two
+      // edge items per lane
       auto flux = [&](int fx_field, int fy_field) {
-        const int nbx = gp_blocks(edges);
+        const int nbx = gp_blocks((edges + 1) / 2);
         if (fuse)
           hipLaunchKernelGGL((gp::k_gp_flux<T, R, true>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
                              fy_field, nbx);

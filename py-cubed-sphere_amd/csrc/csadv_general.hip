@@ -292,7 +292,8 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int 
   const unsigned M = v.M, ne = v.N + 1, total = 6u * M * ne;
   const bool mt0 = D.mt == 1;
   const T *gc = reinterpret_cast<const T *>(D.gc);
-  for (unsigned t = b * blockDim.x + threadIdx.x; t < total; t += nb * blockDim.x) {
+  // flux at edge item t (and its offset in the flux array)
+  auto flux_at = [&](unsigned t, int64_t &fo) -> T {
     // consecutive threads take consecutive columns in both directions (coalesced)
     const unsigned w = dir == 0 ? ne : M;
     const unsigned u = t / w, r = t % w;
@@ -300,7 +301,7 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int 
     const int a = dir == 0 ? (int)(u % M) : (int)r;        // the other index
     const int p = dir == 0 ? (int)(u / M) : (int)(u / ne);
     T uav, usg, ge_l, ge_r, gce, ge;
-    int64_t cl, cr, fo;
+    int64_t cl, cr;
     const T *ge_arr;
     int ge_st;
     int64_t ge_base;
@@ -352,7 +353,21 @@ __global__ void k_gp_flux(const Solve *solves, int s, int tr, int fx_field, int 
              : (qL - cc * T(0.5) * (q6 + dq) - q6 * cc * cc / T(3.0));
     f = f * uav;
     if (!mt0) f = f * ge;
-    (dir == 0 ? v.fx : v.fy)[fo] = f;
+    return f;
+  };
+  // Two edge items per lane and iteration (t and t + stride), both computed before either is
+  // stored so that their loads are in flight together; a lane without a second item
+  // recomputes its first and stores it once.
+  T *out = dir == 0 ? v.fx : v.fy;
+  const unsigned stride = nb * blockDim.x;
+  for (unsigned t = b * blockDim.x + threadIdx.x; t < total; t += 2 * stride) {
+    const unsigned t2 = t + stride;
+    const bool has2 = t2 < total;
+    int64_t fo1, fo2;
+    const T f1 = flux_at(t, fo1);
+    const T f2 = flux_at(has2 ? t2 : t, fo2);
+    out[fo1] = f1;
+    if (has2) out[fo2] = f2;
   }
 }
 
