@@ -1434,7 +1434,7 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
         gp_edges<T, R>(h, st, s, tr, 0, 0);
       }
       flux(0, 0);
-      hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev, s, tr);
       if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
       // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
       if (!fuse) {
@@ -1450,9 +1450,11 @@ int enqueue_general_r(csadv_t *h, hipStream_t st) {
         hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
       } else {
         if (d.mf == 2)
-          hipLaunchKernelGGL((gp::k_gp_div_update<T, true>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+          hipLaunchKernelGGL((gp::k_gp_div_update<T, true>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev,
+                             s, tr);
         else
-          hipLaunchKernelGGL((gp::k_gp_div_update<T, false>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+          hipLaunchKernelGGL((gp::k_gp_div_update<T, false>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev,
+                             s, tr);
       }
     }
     hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);

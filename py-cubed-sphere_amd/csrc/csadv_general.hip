@@ -463,13 +463,22 @@ __global__ void k_gp_split(const Solve *solves, int s, int tr) {
   const csadv_solve_desc &D = v.S->d;
   if (finished(D)) return;
   const unsigned M = v.M, total = 6u * M * M;
-  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const unsigned u = t / M;
+  // two cells per lane (t and t + stride), both computed before any store
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += 2 * stride) {
+    const unsigned t2 = t + stride < total ? t + stride : t;
+    const unsigned u = t / M, u2 = t2 / M;
     const int i = t % M, j = u % M, p = u / M;
-    const int64_t c = cidx(v.pstride, v.ld, p, j, i);
+    const int i2 = t2 % M, j2 = u2 % M, p2 = u2 / M;
     const T Qx = qsplit<T, 0>(v, D, p, j, i), Qy = qsplit<T, 1>(v, D, p, j, i);
+    const T Qx2 = qsplit<T, 0>(v, D, p2, j2, i2), Qy2 = qsplit<T, 1>(v, D, p2, j2, i2);
+    const int64_t c = cidx(v.pstride, v.ld, p, j, i), c2 = cidx(v.pstride, v.ld, p2, j2, i2);
     v.qx[c] = Qx;
     v.qy[c] = Qy;
+    if (t2 != t) {
+      v.qx[c2] = Qx2;
+      v.qy[c2] = Qy2;
+    }
   }
 }
 
@@ -547,10 +556,11 @@ __global__ void k_gp_div_update(const Solve *solves, int s, int tr) {
   if (finished(D)) return;
   const unsigned M = v.M, total = 6u * M * M;
   const T *gc = reinterpret_cast<const T *>(D.gc);
-  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  // new value of cell item t (and its offset)
+  auto next_q = [&](unsigned t, int64_t &c) -> T {
     const unsigned u = t / M;
     const int i = t % M, j = u % M, p = u / M;
-    const int64_t c = cidx(v.pstride, v.ld, p, j, i);
+    c = cidx(v.pstride, v.ld, p, j, i);
     const bool interior = i >= v.i0 && i < v.iend && j >= v.i0 && j < v.iend;
     const T q = v.qin[c];
     if (interior) {
@@ -566,10 +576,18 @@ __global__ void k_gp_div_update(const Solve *solves, int s, int tr) {
       const T g = gc[(int64_t)j * v.ld + i];
       const T dFx = dflux(fe, fw, T(D.dt), T(D.dx)), dFy = dflux(fn, fs, T(D.dt), T(D.dy));
       const T dv = -(dFx + dFy) / (T(D.dt) * g);
-      v.qout[c] = q - T(D.dt) * dv;
-    } else {
-      v.qout[c] = q;
+      return q - T(D.dt) * dv;
     }
+    return q;
+  };
+  // two cells per lane (t and t + stride), both computed before any store
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += 2 * stride) {
+    const bool has2 = t + stride < total;
+    int64_t c1, c2;
+    const T q1 = next_q(t, c1), q2 = next_q(has2 ? t + stride : t, c2);
+    v.qout[c1] = q1;
+    if (has2) v.qout[c2] = q2;
   }
 }
 
