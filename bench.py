@@ -135,15 +135,8 @@ def cpu_baseline(N, case, dt, seconds, opts=None):
 
 def batch_specs(seed, steps):
     """The 64 seeded solves of the batched form: (N, case, steps)."""
-    import numpy as np
-    from cubedsphere.cases import random_solid_body
-    rng = np.random.default_rng(seed)
-    specs = []
-    for _ in range(64):
-        N = int(rng.choice(BATCH_NS))
-        ntr = int(rng.integers(1, 9))
-        specs.append((N, random_solid_body(rng, ntr), steps))
-    return specs
+    from cubedsphere.batch import seeded_batch
+    return seeded_batch(seed, steps, BATCH_NS)
 
 
 def timed_run(stepper, K, W, graph):

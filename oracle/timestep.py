@@ -54,6 +54,11 @@ class Simulation:
         self.dt, self.Tf = dt, Tf
         self.dto2, self.twodt = dt * 0.5, dt * 2.0
         self.case = case
+        # the reference object's own option attributes (src/advection_ic.py:24-49, :124-168)
+        self.ic, self.vf, self.tc = case.get("ic"), case.get("vf"), 1
+        self.recon, self.dp, self.opsplit = recon, dp, opsplit
+        self.edge_treatment, self.metric_tensor, self.mass_fixer = et, mt, mf
+        self.degree = 3
         self.recon_name, self.dp_name = RECON[recon], DP[dp]
         self.opsplit_name, self.et_name = SPLIT[opsplit], ET[et]
         self.mt_name, self.mf_name = MT[mt], MF[mf]

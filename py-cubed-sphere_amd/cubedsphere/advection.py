@@ -6,23 +6,26 @@ src/advection_error.py:21-188, main.py test case 5) with the step on the GPU:
   sim = adv_simulation_par(cs_grid, dt, Tf, ic, vf, tc, recon, dp, opsplit, et, mt, mf)
   init_vars_adv(cs_grid, sim)            # src/advection_vars.py:19
   adv_time_step(cs_grid, sim, k, t)      # src/advection_timestep.py:19  (one step on the device)
-  update_adv(cs_grid, sim, t)            # src/advection_timestep.py:48  (no-op: see below)
+  update_adv(cs_grid, sim, t)            # src/advection_timestep.py:48  (winds of the next step)
   adv_sphere(cs_grid, sim) -> (linf, l1, l2)
   error_analysis_adv(sim) -> errors table of the convergence study (tc = 2), rates as the reference's
   output_adv(cs_grid, sim, k, nsteps)    # src/output.py: errors, mass, lat-lon remaps, data files
   main(pardir)                           # test case 5 from par/configuration.par + advection.par
 
-Differences of contract, all deliberate: ``cs_grid`` is a :class:`cubedsphere.grid.Grid`;
-``sim`` is a :class:`cubedsphere.solver.Advection` whose state lives on the GPU (read it
-with ``sim.state()``); the time-dependent winds of ``update_adv`` are recombined on the
-device at the start of the next step, so ``update_adv`` has nothing left to do.  The output
-side (``output_adv``) computes the errors, mass, the device lat-lon remaps and the reference's
-data files; it draws no plots (cartopy is not available).  Errors are those of
-src/errors.py:99-113 at the final step.
+``init_vars_adv``, ``adv_time_step`` and ``update_adv`` take either this module's objects (a
+:class:`cubedsphere.grid.Grid` and the :class:`cubedsphere.solver.Advection` that
+``adv_simulation_par`` returns, state read with ``sim.state()``) or the reference's own
+``cubed_sphere`` and ``adv_simulation_par`` objects, whose ``Q`` then reads the device state
+(:mod:`cubedsphere.seam`): the reference's own ``adv_sphere`` loop runs unchanged with these
+three imported from here.  Time-dependent winds are recombined on the device at the start of
+every step from basis fields and one coefficient row per step; ``update_adv`` sets the row of
+the next step.  The output side (``output_adv``) computes the errors, mass, the device lat-lon
+remaps and the reference's data files; it draws no plots (cartopy is not available).  Errors
+are those of src/errors.py:99-113 at the final step.
 """
 import numpy as np
 
-from . import config
+from . import config, seam
 from ._lib import CsadvError
 from .cases import reference_case
 from .grid import Grid
@@ -49,7 +52,14 @@ def adv_simulation_par(cs_grid, dt, Tf, ic, vf, tc, recon, dp, opsplit, et, mt, 
 def init_vars_adv(cs_grid, simulation):
     """src/advection_vars.py:19-107: initial condition, winds, tables, the device handle, and
     the CFL number of :88-98 (the largest signed Courant number of the interior x-edges of all
-    rows and the interior y-edges of all columns, in absolute value)."""
+    rows and the interior y-edges of all columns, in absolute value).
+
+    ``simulation`` is either an :class:`Advection` (from :func:`adv_simulation_par` here) or the
+    reference's own ``adv_simulation_par`` object with its ``cubed_sphere`` grid: the latter is
+    bound to a device solve (:mod:`cubedsphere.seam`) and its ``Q`` then reads the device state."""
+    if not isinstance(simulation, Advection):
+        seam.bind(cs_grid, simulation)
+        return
     simulation.init_vars_adv()
     simulation.total_mass0 = simulation.mass()
     M, i0, iend = cs_grid.M, cs_grid.i0, cs_grid.iend
@@ -62,16 +72,30 @@ def init_vars_adv(cs_grid, simulation):
 
 def adv_time_step(cs_grid, simulation, k, t):
     """src/advection_timestep.py:19-43: one step (ghost fill, winds, split fluxes, update,
-    mass fixer).  ``k`` and ``t`` must follow the run (the device tracks the step count)."""
+    mass fixer).  ``k`` and ``t`` must follow the run (the device tracks the step count).  For
+    the reference's own objects the step ignores ``k`` and ``t``, as the reference's does."""
+    if not isinstance(simulation, Advection):
+        seam.step(cs_grid, simulation, k, t)
+        return
     if k != simulation.steps_done + 1:
         raise CsadvError("adv_time_step: step %d requested after %d steps" % (k, simulation.steps_done))
     simulation.step(1)
 
 
 def update_adv(cs_grid, simulation, t):
-    """src/advection_timestep.py:48-75.  The device recombines the time-dependent winds at the
-    start of every step (from precomputed basis fields), so nothing is left to update here."""
-    return None
+    """src/advection_timestep.py:48-75: the next step's winds become the analytic wind at ``t``.
+    The device recombines the time-dependent winds from precomputed basis fields at the start
+    of every step, with one coefficient row per step; this writes the row for time ``t`` when it
+    differs from the one of the uniform times ``t = k dt`` (which it never does inside
+    :func:`adv_sphere`) and grows the table past its end."""
+    if not isinstance(simulation, Advection):
+        seam.update(cs_grid, simulation, t)
+        return
+    if simulation.case.steady:
+        return
+    prev = getattr(simulation, "_wind_t", 0.0)
+    simulation.set_wind_times(float(t), prev)
+    simulation._wind_t = float(t)
 
 
 def adv_sphere(cs_grid, simulation, divtest_flag=False, datadir=None, verbose=False):

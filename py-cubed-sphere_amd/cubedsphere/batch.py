@@ -86,13 +86,23 @@ class Batch:
             self._open(list(fresh))
 
     def step(self, nsteps=1):
+        grew = False
         for m in self.members:
             todo = nsteps if not m.stop else max(0, min(nsteps, m.stop - m.steps_done))
-            if not m.case.steady and m.steps_done + todo > m.max_steps:
-                raise _lib.CsadvError("unsteady winds: coefficient table holds %d steps" % m.max_steps)
+            grew = m.ensure_steps(m.steps_done + todo, refresh=False) or grew
+        if grew:
+            self.refresh()
         _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
         for m in self.members:
             m.steps_done += nsteps if not m.stop else max(0, min(nsteps, m.stop - m.steps_done))
+
+    def refresh(self):
+        """Hand the members' changed buffer pointers (a grown wind table) to the handle."""
+        if not self.handle:
+            return
+        descs = (_lib.SolveDesc * len(self.members))(*[m.desc() for m in self.members])
+        _lib.check(self.lib.csadv_update(self.handle, descs, len(self.members), self._stream()), self.handle,
+                   "csadv_update")
 
     def cell_updates_per_step(self):
         return sum(6 * m.grid.N ** 2 * m.case.ntr for m in self.members)
@@ -150,3 +160,21 @@ def make_members(specs, dtype="f32", cfl=0.5, device="cuda:0", tiles=None, devic
     ({N: tile}; default 0)."""
     return [make_member(N, case, nsteps, dtype=dtype, cfl=cfl, device=device, tile=(tiles or {}).get(N, 0),
                         device_grid=device_grid, **opts) for N, case, nsteps in specs]
+
+
+BATCH_NS = (32, 48, 96, 192, 384, 512)
+
+
+def seeded_batch(seed, steps, sizes=BATCH_NS, count=64):
+    """The seeded solves of the batched form: ``count`` specs (N, case, steps), N drawn from
+    ``sizes``, 1-8 tracers each, Gaussian centres uniform on the sphere, width b in [5, 20] and
+    rotation-axis tilt in [0, pi/2] (cases.random_solid_body)."""
+    import numpy as np
+    from .cases import random_solid_body
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        N = int(rng.choice(sizes))
+        ntr = int(rng.integers(1, 9))
+        specs.append((N, random_solid_body(rng, ntr), steps))
+    return specs

@@ -44,6 +44,7 @@ class Completed:
     done: float
     cell_updates: float
     result: np.ndarray = field(repr=False)
+    dt: float = 0.0         # time step the solve ran with (CFL 0.5 at t = 0)
 
     @property
     def latency(self):
@@ -198,7 +199,8 @@ class Server:
                     res = m.state()
                     t = self.clock() - t0
                     req = m.request
-                    c = Completed(req.rid, req.arrival, m.admitted, t, 6.0 * req.N ** 2 * req.case.ntr * req.nsteps, res)
+                    c = Completed(req.rid, req.arrival, m.admitted, t, 6.0 * req.N ** 2 * req.case.ntr * req.nsteps, res,
+                                  m.dt)
                     done[req.rid] = c
                     if on_complete:
                         on_complete(c)

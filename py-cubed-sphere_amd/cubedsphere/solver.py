@@ -304,6 +304,8 @@ class Advection:
         self.corr.zero_()
         torch.cuda.synchronize(self.device)
         self.steps_done = 0
+        self._wind_t = 0.0            # time of the last wind update (update_adv)
+        self._wind_rows = {}          # coefficient rows written for non-uniform update times
         return self
 
     # ------------------------------------------------------------ stepping
@@ -313,10 +315,69 @@ class Advection:
             raise _lib.CsadvError("init_vars_adv() first")
         if self.owner is not None:
             raise _lib.CsadvError("this solve is stepped by its batch (Batch.step)")
-        if not self.case.steady and self.steps_done + nsteps > self.max_steps:
-            raise _lib.CsadvError("unsteady winds: coefficient table holds %d steps" % self.max_steps)
+        self.ensure_steps(self.steps_done + nsteps)
         _lib.check(self.lib.csadv_step(self.handle, int(nsteps), self._stream()), self.handle, "csadv_step")
         self.steps_done += nsteps
+
+    # ------------------------------------------------------------ time-dependent winds
+    def ensure_steps(self, total, refresh=True):
+        """Grow the wind-coefficient table of a time-dependent wind so that it covers ``total``
+        steps (rows of t_n = n dt, the times src/advection_sphere.py:45-57 passes to update_adv).
+        Steady winds need one row and never grow.  Rows already written (update_adv at other
+        times) are kept.  Returns True when the table grew (the handle then needs ``refresh``,
+        which this call makes unless ``refresh`` is False)."""
+        if self.case.steady or total <= self.max_steps:
+            return False
+        new = max(int(total), 2 * self.max_steps)
+        tab = torch.from_numpy(self.case.coefficient_table(new, self.dt).reshape(-1)).to(self.coef)
+        tab[:self.coef.numel()] = self.coef          # keep rows set from update_adv
+        self.coef = tab
+        self.max_steps = new
+        if refresh:
+            self.refresh()
+        return True
+
+    def set_wind_times(self, t, t_prev):
+        """Winds of the next step from the analytic wind at ``t`` (and ``t_prev``, the time of
+        the previous wind update, for RK2's extrapolation): what update_adv leaves behind for the
+        next adv_time_step (src/advection_timestep.py:48-75).  Writes one coefficient row only
+        when it differs from the row already there (at the uniform times t = n dt it never does)."""
+        if self.case.steady:
+            return
+        n = self.steps_done
+        self.ensure_steps(n + 1)
+        row = np.stack([self.case.coefficients(t), self.case.coefficients(t_prev)]).reshape(-1)
+        rows = self._wind_rows
+        have = rows.get(n)
+        if have is None:     # the row coefficient_table wrote: (s(t_n), s(t_(n-1)))
+            have = np.stack([self.case.coefficients(n * self.dt),
+                             self.case.coefficients(max(n - 1, 0) * self.dt)]).reshape(-1)
+        if not np.array_equal(have, row):
+            self.coef[n * row.size:(n + 1) * row.size].copy_(torch.from_numpy(row))
+            rows[n] = row
+
+    def refresh(self):
+        """Hand changed buffer pointers to the library (the handle's descriptor copy)."""
+        if self.handle is None:
+            return
+        if self.owner is not None:
+            self.owner.refresh()
+            return
+        d = self.desc()
+        _lib.check(self.lib.csadv_update(self.handle, ctypes.byref(d), 1, self._stream()), self.handle,
+                   "csadv_update")
+
+    def load_state(self, q):
+        """Replace the interior of every tracer with ``q`` ([ntr][6][N][N] or [6][N][N] for one
+        tracer, FP64 or the solve's dtype): the pending mass-fixer totals are folded first, so the
+        device then holds exactly ``q``."""
+        self.finalize()
+        g = self.grid
+        I = slice(g.i0, g.iend)
+        t = torch.as_tensor(np.asarray(q) if not torch.is_tensor(q) else q)
+        if t.dim() == 3:
+            t = t.unsqueeze(0)
+        self.q[self.current()][:, :, I, I] = t.to(self.q[0].device, self.q[0].dtype)
 
     def use_graph(self, steps_per_graph):
         """Replay groups of ``steps_per_graph`` steps from one captured HIP graph (0 = off)."""

@@ -1,0 +1,232 @@
+"""The benchmark configurations at their stated sizes, against the FP64 oracle.
+
+  polar N=48       one full revolution (384 steps) of the polar solid-body rotation, FP64
+  NL10 N=768       4 tracers in the deformational wind: FP64 vs the oracle after 2 steps; FP32 vs
+                   the FP64 oracle at N=96; FP32 vs FP64 at N=768 over 300 steps and the whole
+                   period T = 5
+  seeded batch     the 64 solves of the batched form: every tracer FP32 vs FP64, the members
+                   with N <= 96 vs the oracle
+  served stream    requests drawn from the served form's ranges (N 48-1024): each result
+                   bitwise the same alone and in the mix, FP64 requests vs the oracle
+
+Bars: FP64 within 1e-12 relative (max |a - b| / max |b|) of the oracle; FP32 within 1e-5
+relative L2 (||q32 - q64|| / ||q64|| per tracer) of FP64.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import interior_dev, oracle_case, rel_max, run_oracle
+from oracle import timestep as ots
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device visible")
+
+
+def _l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _solver(N, case, dt, nsteps, **kw):
+    from cubedsphere.solver import Advection
+    return Advection(N, case, dt, device="cuda:0", max_steps=nsteps, **kw).init_vars_adv()
+
+
+def test_polar_n48_full_revolution():
+    """N=48, Gaussian (b=10) at (3pi/2, 0), polar rotation of period 12 days, CFL 0.5: 384 steps
+    of dt = 2700 s close one revolution.  State, mass and final norms against the oracle."""
+    from cubedsphere.cases import workload
+    from cubedsphere.grid import Grid
+    from cubedsphere.solver import cfl_dt
+    N = 48
+    case = workload("solid_body_polar")
+    dt = cfl_dt(Grid(N), case, 0.5)
+    nsteps = int(round(12.0 * 86400.0 / dt))
+    assert nsteps == 384 and abs(dt - 2700.0) < 1e-9
+    sol = _solver(N, case, dt, nsteps)
+    m0 = sol.mass()
+    sol.step(nsteps)
+    grid, sim = ots.make(N, oracle_case(case), dt)
+    ref_norms = ots.run(grid, sim, nsteps)
+    assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
+    got = sol.norms(t=nsteps * dt)
+    for a, b in zip(got, ref_norms):
+        assert abs(a - b) <= 1e-10 * abs(b)
+    ref_m, _ = ots.mass_computation(sim.Q, grid, 1.0)
+    assert abs(sol.mass() - ref_m) <= 1e-13 * abs(ref_m)
+    assert abs(sol.mass() - m0) <= 1e-13 * abs(m0)
+
+
+def _nl10(N):
+    from cubedsphere.cases import workload
+    from cubedsphere.grid import Grid
+    from cubedsphere.solver import cfl_dt
+    case = workload("nl10_deformation")
+    dt_cfl = cfl_dt(Grid(N), case, 0.5)
+    nper = int(math.ceil(5.0 / dt_cfl))          # steps of one period T = 5 at CFL <= 0.5
+    return case, 5.0 / nper, nper
+
+
+def test_nl10_n768_fp64_matches_oracle():
+    N, nsteps = 768, 2
+    case, dt, _ = _nl10(N)
+    sol = _solver(N, case, dt, nsteps)
+    sol.step(nsteps)
+    got = sol.state()
+    sol.close()
+    grid, sim = run_oracle(N, case, dt, nsteps)
+    for n in range(case.ntr):
+        assert rel_max(got[n], interior_dev(grid, sim.Qs[n])) < TOL64, n
+
+
+def test_nl10_fp32_against_fp64_oracle():
+    """The precision form against the FP64 oracle itself (not the FP64 device run)."""
+    N, nsteps = 96, 60
+    case, dt, _ = _nl10(N)
+    sol = _solver(N, case, dt, nsteps, dtype="f32")
+    sol.step(nsteps)
+    got = sol.state()
+    grid, sim = run_oracle(N, case, dt, nsteps)
+    for n in range(case.ntr):
+        l2 = _l2(got[n], interior_dev(grid, sim.Qs[n]))
+        print("N=96 FP32 vs oracle tracer %d: L2 %.3e" % (n, l2))
+        assert l2 <= TOL32, (n, l2)
+
+
+def _fp32_fp64(N, nsteps, dt, case):
+    out, mass = {}, {}
+    for dtype in ("f64", "f32"):
+        sol = _solver(N, case, dt, nsteps, dtype=dtype)
+        m0 = [sol.mass(n) for n in range(case.ntr)]
+        sol.use_graph(16)
+        sol.step(nsteps)
+        out[dtype] = sol.state()
+        mass[dtype] = [abs(sol.mass(n) - m0[n]) / abs(m0[n]) for n in range(case.ntr)]
+        sol.close()
+    return out, mass
+
+
+def test_nl10_n768_fp32_vs_fp64_300_steps():
+    N, nsteps = 768, 300
+    case, dt, _ = _nl10(N)
+    out, mass = _fp32_fp64(N, nsteps, dt, case)
+    for n in range(case.ntr):
+        l2 = _l2(out["f32"][n], out["f64"][n])
+        print("N=768 FP32 vs FP64 after %d steps, tracer %d: L2 %.3e, mass drift %.2e" % (nsteps, n, l2, mass["f32"][n]))
+        assert l2 <= TOL32, (n, l2)
+        assert mass["f32"][n] <= 1e-6 and mass["f64"][n] <= 1e-12
+
+
+def test_nl10_n768_fp32_vs_fp64_whole_period():
+    """The whole precision-form run: T = 5 (one period of the deformational wind, about 16.6k
+    steps).  Reports the FP32-vs-FP64 L2 per tracer and the error of each against the exact
+    solution (the initial field, which the wind returns at t = T)."""
+    N = 768
+    case, dt, nsteps = _nl10(N)
+    out, mass = _fp32_fp64(N, nsteps, dt, case)
+    from cubedsphere.grid import Grid
+    g = Grid(N)
+    I = slice(g.i0, g.iend)
+    worst = 0.0
+    for n in range(case.ntr):
+        exact = case.q(n, g.pc.lon[:, I, I], g.pc.lat[:, I, I], 5.0)
+        l2 = _l2(out["f32"][n], out["f64"][n])
+        e32, e64 = _l2(out["f32"][n], exact), _l2(out["f64"][n], exact)
+        worst = max(worst, l2)
+        print("N=768 T=5 (%d steps) tracer %d: FP32 vs FP64 L2 %.3e; vs exact FP32 %.4e FP64 %.4e; "
+              "FP32 mass drift %.2e" % (nsteps, n, l2, e32, e64, mass["f32"][n]))
+        assert l2 <= TOL32, (n, l2)
+        # the FP32 rounding stays far below the scheme's own error over the whole run
+        assert abs(e32 - e64) <= 1e-2 * e64, (n, e32, e64)
+        assert mass["f32"][n] <= 1e-5
+
+
+def _batch_specs(steps):
+    from cubedsphere.batch import seeded_batch
+    return seeded_batch(2024, steps)
+
+
+def test_seeded_batch_fp32_vs_fp64_every_tracer():
+    from cubedsphere.batch import Batch, make_members
+    nsteps = 50
+    specs = _batch_specs(nsteps)
+    res = {}
+    for dtype in ("f64", "f32"):
+        b = Batch(make_members(specs, dtype=dtype, device_grid=True), graph_steps=16).init()
+        b.step(nsteps)
+        res[dtype] = [m.state() for m in b.members]
+        b.close()
+    worst = 0.0
+    for s, (a, b) in enumerate(zip(res["f32"], res["f64"])):
+        for n in range(a.shape[0]):
+            l2 = _l2(a[n], b[n])
+            worst = max(worst, l2)
+            assert l2 <= TOL32, (s, n, l2)
+    print("seeded batch: 256 tracers, worst FP32 vs FP64 L2 %.3e after %d steps" % (worst, nsteps))
+    assert sum(a.shape[0] for a in res["f32"]) == 256
+
+
+def test_seeded_batch_small_members_match_oracle():
+    from cubedsphere.batch import Batch, make_members
+    nsteps = 10
+    specs = _batch_specs(nsteps)
+    b = Batch(make_members(specs, dtype="f64", device_grid=True), graph_steps=0).init()
+    b.step(nsteps)
+    checked = 0
+    for m in b.members:
+        if m.grid.N > 96:
+            continue
+        grid, sim = run_oracle(m.grid.N, m.case, m.dt, nsteps)
+        got = m.state()
+        for n in range(m.case.ntr):
+            assert rel_max(got[n], interior_dev(grid, sim.Qs[n])) < TOL64, (m.grid.N, n)
+        checked += 1
+    b.close()
+    assert checked == 29
+
+
+def _served_stream():
+    """Requests drawn from the served form's ranges (N log-uniform in [48, 1024], 1-4 tracers,
+    Gaussian or cap fields, solid-body or deformational wind), steps capped at 300, arriving
+    together so they share rounds; two small FP64 requests are appended for the oracle."""
+    from cubedsphere.serve import Request, random_case, random_requests
+    reqs = random_requests(31, 8, rate=500.0, n_range=(48, 1024), tracers=(1, 4), steps=(50, 300),
+                           f64_fraction=0.25)
+    rng = np.random.default_rng(3)
+    t = reqs[-1].arrival
+    reqs.append(Request(100, t, 48, random_case(rng, 2), 120, "f64"))
+    reqs.append(Request(101, t, 72, random_case(rng, 1), 90, "f64"))
+    return reqs
+
+
+def test_served_stream_alone_equals_mix_and_fp64_matches_oracle():
+    from cubedsphere.serve import Server, serve_alone
+    reqs = _served_stream()
+    assert max(r.N for r in reqs) > 512
+    srv = Server(chunk=16, max_slots=6)
+    done = srv.run(reqs)
+    assert sorted(done) == sorted(r.rid for r in reqs)
+    assert srv.stats["peak_slots"] > 1
+    for r in reqs:
+        alone = serve_alone(r, chunk=16)
+        assert np.array_equal(done[r.rid].result, alone), r.rid
+    for r in reqs:
+        if r.dtype != "f64" or r.N > 96:
+            continue
+        grid, sim = ots.make(r.N, oracle_case(r.case), done[r.rid].dt)
+        ots.init_vars_adv(grid, sim)
+        for k in range(1, r.nsteps + 1):
+            ots.adv_time_step(grid, sim, k, k * sim.dt)
+            ots.update_adv(grid, sim, k * sim.dt)
+        for n in range(r.case.ntr):
+            assert rel_max(done[r.rid].result[n], interior_dev(grid, sim.Qs[n])) < TOL64, (r.rid, n)

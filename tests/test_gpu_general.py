@@ -92,3 +92,51 @@ def test_general_step_graph_replay_bitwise():
     a.step(10)
     b.step(10)
     assert np.array_equal(a.full_state(), b.full_state())
+
+
+# the smallest N the library takes (8: the three-cell extrapolation windows of opposite panel
+# sides and the flux predicate ec <= i0 + 1 || ec >= iend - 2 come closest) and odd N (a cell
+# centre on each pole)
+SMALL_N_OPTS = [
+    dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),        # ET-PL07, PL07 recon
+    dict(recon=2, opsplit=1, et=2, mt=1, dp=1, mf=3),        # ET-PL07, CW84 recon, MF-PR
+    dict(recon=3, opsplit=1, et=3, mt=1, dp=2, mf=2),        # MF-AF
+    dict(recon=4, opsplit=2, et=1, mt=1, dp=2, mf=2),        # ET-S72 + MF-AF, RK2 wind copy
+]
+
+
+@pytest.mark.parametrize("N", [8, 9, 17])
+@pytest.mark.parametrize("opts", SMALL_N_OPTS, ids=lambda o: "-".join("%s%d" % kv for kv in sorted(o.items())))
+def test_general_step_small_and_odd_n(N, opts):
+    from cubedsphere.cases import reference_case
+    dt, nsteps = 0.4 * (np.pi / 2 / N) / (2 * np.pi / 5 * 1.42), 6
+    case = reference_case(2, 1)
+    sol = _solve(N, case, dt, nsteps, **opts)
+    grid, sim = run_oracle(N, case, dt, nsteps, **opts)
+    assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
+
+
+@pytest.mark.parametrize("opts", [dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),
+                                  dict(recon=3, opsplit=1, et=1, mt=1, dp=1, mf=3),
+                                  dict(recon=3, opsplit=1, et=3, mt=1, dp=2, mf=2)],
+                         ids=["etpl07", "ets72-mfpr", "rk2-mfaf"])
+def test_general_step_fp32_close_to_fp64(opts):
+    """The precision form through the general step: FP32 within 1e-5 relative L2 of FP64 after
+    200 steps, and the mass of the conservative options kept to FP32 round-off."""
+    from cubedsphere.cases import Case
+    N, nsteps = 48, 200
+    case = Case([{"field": "gaussians", "centres": [(0.7, 0.3)], "b": 20.0}], ("solid_body", -0.6, 5.0), "narrow")
+    dt = 0.5 * (np.pi / 2 / N) / (2 * np.pi / 5 * 1.42)
+    out, drift = {}, {}
+    from cubedsphere.solver import Advection
+    for dtype in ("f32", "f64"):
+        sol = Advection(N, case, dt, device="cuda:0", dtype=dtype, **opts).init_vars_adv()
+        m0 = sol.mass()
+        sol.step(nsteps)
+        out[dtype] = sol.state()[0]
+        drift[dtype] = abs(sol.mass() - m0) / abs(m0)
+    l2 = np.linalg.norm(out["f32"] - out["f64"]) / np.linalg.norm(out["f64"])
+    print("general step FP32 %s: L2 vs FP64 %.3e, mass drift FP32 %.2e FP64 %.2e" % (opts, l2, drift["f32"], drift["f64"]))
+    assert l2 <= 1e-5, l2
+    if opts["mf"] != 1:
+        assert drift["f64"] <= 1e-13 and drift["f32"] <= 2e-6, drift

@@ -225,3 +225,15 @@ def test_drop_in_driver_matches_oracle_run(tmp_path):
     ref = ots.run(grid, sim, 10)
     for a, b in zip((linf, l1, l2), ref):
         assert abs(a - b) <= 1e-10 * abs(b)
+
+
+@pytest.mark.parametrize("N", [17, 33])
+@pytest.mark.parametrize("opts", [dict(), dict(recon=2, opsplit=2, dp=2, mf=1)], ids=["default", "cw84-l04-rk2"])
+def test_odd_n_matches_oracle(N, opts):
+    """Odd N puts a cell centre on each pole (panels 4 and 5), where lon = atan2(+-0, +-0)."""
+    dt, nsteps = 0.5 * (math.pi / 2 / N) / (2 * math.pi / 5 * 1.42), 8
+    case = _ref(2, 1)
+    sol = _solver(N, case, dt, **opts)
+    sol.step(nsteps)
+    grid, sim = run_oracle(N, case, dt, nsteps, **opts)
+    assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
