@@ -2,7 +2,8 @@
 
 Restates src/edges_treatment.py:31-347:
   average_parabola_cube_edges :31-76    (ET-PL07 parabola averaging, 12 cube edges)
-  edges_extrapolation         :82-224   (ET-PL07, PL07 appendix C eqs. 47 and 49)
+  edges_extrapolation         :82-178   (ET-PL07, PL07 appendix C eqs. 47 and 49)
+  ghost_parabolas             :119-178  (its ghost-cell copies of the averaged parabolas)
   average_flux_cube_edges     :231-278  (MF-AF)
   vector_edges_rk2_copy       :305-347  (ET-S72 / ET-PL07 wind halo for RK2)
 The statement order is kept exactly: later statements read values written by
@@ -65,8 +66,13 @@ def edges_extrapolation(Qx, Qy, px, py, g, recon_name):
         py.q_L[I, jend - 2, :] = py.q_R[I, jend - 3, :]
 
     average_parabola_cube_edges(px, py, g)
+    ghost_parabolas(px, py, g)
 
-    # ghost-cell copies of the averaged parabolas (:119-178)
+
+def ghost_parabolas(px, py, g):
+    """src/edges_treatment.py:119-178: ghost-cell copies of the averaged parabolas."""
+    i0, j0, iend, jend = g.i0, g.j0, g.iend, g.jend
+    J, I = slice(j0, jend), slice(i0, iend)
     px.q_L[iend, J, 0:3] = px.q_L[i0, J, 1:4]
     px.q_R[iend, J, 0:3] = px.q_R[i0, J, 1:4]
     px.q_L[i0 - 1, J, 1:4] = px.q_L[iend - 1, J, 0:3]

@@ -38,11 +38,10 @@ def G_operator(grid, sim):
     sim.py.dF[:, j0:jend] = sim.py.dF[:, j0:jend] * sim.dt / grid.dy
 
 
-def divergence(grid, sim):
-    """src/discrete_operators.py:18-101"""
-    i0, iend, j0, jend = grid.i0, grid.iend, grid.j0, grid.jend
+def split_half_steps(grid, sim):
+    """src/discrete_operators.py:31-73: the first flux pass on Q and the split's inner
+    half-steps (Qx, Qy), divided by the metric under MT-0."""
     N, ng = grid.N, grid.ng
-    dt = sim.dt
     g = grid.metric_tensor_pc
 
     sim.gQ[:] = sim.Q * g
@@ -78,6 +77,15 @@ def divergence(grid, sim):
         Qx, Qy = gQx / g, gQy / g
     elif sim.opsplit_name != "SP-PL07":
         raise NameError("MT-PL07 needs SP-PL07 (src/discrete_operators.py:72-82)")
+    return Qx, Qy
+
+
+def divergence(grid, sim):
+    """src/discrete_operators.py:18-101"""
+    i0, iend, j0, jend = grid.i0, grid.iend, grid.j0, grid.jend
+    dt = sim.dt
+    g = grid.metric_tensor_pc
+    Qx, Qy = split_half_steps(grid, sim)
 
     if sim.et_name in ("ET-S72", "ET-PL07"):
         ghost_cells_adjacent_panels(Qx, Qy, grid)
