@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CSADV_ABI_VERSION 5
+#define CSADV_ABI_VERSION 6
 
 enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
 
@@ -88,7 +88,9 @@ typedef struct csadv_solve_desc {
   const double *coef;              /* [max_steps][2][nbasis] basis coefficients (t_{n-1}, t_{n-2}) */
   double *corr;                    /* [2][ntr] MF-PR correction totals C (FP64), slot by step parity: Q = S + C gfix */
   double *partials;                /* scratch, >= ntr * csadv_partials_per_tracer()     */
-  int32_t *ctl;                    /* [4] device control words: cur, step, 0, 0          */
+  int32_t *ctl;                    /* [4] device control words, zero at init: halt flag
+                                      (csadv_diagnostics), steps completed, running step,
+                                      mass-fixer partials pending                          */
   int32_t tile;                    /* step tile shape of this solve (csadv_tile_rows);
                                       -1 = the handle's tile.  The per-wavefront mass-fixer
                                       partial sums follow the tile, so a solve keeps the same
@@ -155,12 +157,29 @@ int csadv_set_graph(csadv_t *h, int32_t steps_per_graph);
 int csadv_finalize(csadv_t *h, int32_t solve, void *stream);
 
 /* Total mass sum_int Q g dx dy of one tracer with a compensated FP64 sum
- * (src/diagnostics.py:14-26).  Synchronises the stream and writes host out[0]. */
+ * (src/diagnostics.py:14-26): csadv_diagnostics, then a copy to host out[0].  Synchronises. */
 int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *stream);
 
 /* Error norms of one tracer against qexact (device, [6][N][N] FP64, i fastest):
  * host out = {Linf, L1, L2} as src/errors.py:99-113 (unweighted means).  Synchronises. */
 int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact, double *out, void *stream);
+
+/* Diagnostics of tracers tracer0 .. tracer0 + ntracers - 1 of one solve into device memory,
+ * without host synchronisation or allocation (a graph may capture it): out ([ntracers][4] device
+ * doubles) = {mass, Linf, L1, L2} per tracer -- the compensated FP64 mass of
+ * src/diagnostics.py:14-26 and the norms of src/errors.py:99-113 against qexact (device
+ * [ntracers][6][N][N] FP64), or of the field itself when qexact is NULL.  Reductions run over
+ * wavefront shuffles in a fixed order (the same bits on every call).  blowup_limit > 0: a Linf above it
+ * halts the solve -- its step kernels leave it untouched from then on (src/output.py:44-47 stops
+ * the run when Linf > 100); csadv_status reports the halt.  Replaces mass_computation +
+ * compute_errors + the blow-up test of output_adv. */
+int csadv_diagnostics(csadv_t *h, int32_t solve, int32_t tracer0, int32_t ntracers, const double *qexact,
+                      double blowup_limit, double *out, void *stream);
+
+/* Steps completed and the halt flag of one solve, read from the device (synchronises the stream);
+ * the host's count of completed steps (csadv_current) follows the device after a halt.  No
+ * counterpart in the reference (its loop exits on a blow-up). */
+int csadv_status(csadv_t *h, int32_t solve, int32_t *steps_done, int32_t *halted, void *stream);
 
 /* Nearest-cell remap of one tracer onto the plotting lat-lon grid (src/interpolation.py:38-149,
  * src/cs_datastruct.py:569-582): point (a, b) is lon = linspace(-pi, pi, nlon)[a],

@@ -15,18 +15,15 @@
 //                                     stage of src/discrete_operators.py:18-101
 //   k_wind_e2c / k_wind_c2e           duo-grid vector ghost fill (src/interpolation.py:347-532)
 //   k_wind_avg                        time-averaged winds (src/averaged_velocity.py:14-62)
-//   k_settle / k_finalize / k_clear_corr, k_mass / k_norms   state read-out and diagnostics
-#include <hip/hip_runtime.h>
+//   k_settle / k_finalize / k_clear_corr, k_diag / k_diag_final   state read-out and diagnostics
+// The fused step kernel lives in csadv_step.hpp (instantiated by step_<dtype>_r<recon>.hip), the
+// general step in csadv_general.hpp (general_<dtype>.hip), the host-side planning in
+// csadv_plan.cpp.
+#include "csadv_internal.hpp"
 
-#include <algorithm>
-#include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <string>
-#include <type_traits>
-#include <vector>
 
-#include "csadv_device.hpp"
 
 using namespace csadv;
 
@@ -192,6 +189,8 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
       "s"(strip_tgt), "s"(corner_src), "s"(corner_tgt), "s"(strip_w), "s"(corner_w), "s"(last_block));
   if (tr >= ntr) return;
   const int steps = ctl[1];            // requested before the recipes, whose wait it would join
+  const int halted = ctl[0];           // set by a diagnostics call whose blow-up limit was exceeded
+  const bool done = (stop > 0 && steps >= stop) || halted != 0;
   if (blockIdx.x == last_block) {      // the correction total after the last step
     if (mf != 3) return;
     // the partials and both stored totals are requested with the control words, not after
@@ -199,7 +198,7 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
     const double c0 = corr[tr], c1 = corr[ntr + tr];
     const int pending = ctl[3];
     const double sum = partial_sum(partials + (int64_t)tr * part_n, part_n, red);
-    if (stop > 0 && steps >= stop) return;
+    if (done) return;
     if (threadIdx.x == 0 && pending) corr[(steps & 1) * ntr + tr] = corr_total(dt, a2, c0, c1, sum, 1, steps);
     return;
   }
@@ -238,8 +237,8 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
     }
   }
   const bool lead = blockIdx.x == 0 && tr == 0 && threadIdx.x == 0;
-  if (stop > 0 && steps >= stop) {          // solve finished: k_step reads ctl[2] and skips it too
-    if (lead) ctl[2] = steps;
+  if (done) {       // solve finished (k_step reads ctl[2] and skips it too) or halted (ctl[2] < 0)
+    if (lead) ctl[2] = halted ? -1 : steps;
     return;
   }
   T *Qt = reinterpret_cast<T *>((steps & 1) ? q1 : q0) + tr * tstride;
@@ -291,361 +290,6 @@ template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
   __shared__ double red[BLOCK];
   ghost_cells<T>(one, red);
-}
-
-// Work item = (solve, panel, tracer, strip, segment): one wavefront marches down strip_w interior
-// columns (lane l <-> column ti-3+l, so the 3-cell halo of each side rides in the outer
-// lanes) over seg_l interior rows.  Iteration y:
-//   * column stage at y-edge e = y-3 from the 6-row windows (rows y-6 .. y-1): y-fluxes of
-//     Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by DPP lane shifts), update of
-//     row y-4 and its mass-fixer partial sum;
-//   * row stage for row y: x-flux of Q, split half-step Qx.
-// The windows are rings indexed by the iteration's phase modulo 6 (the loop is unrolled by
-// six, so every slot is a fixed register and nothing is shifted); row quantities that the
-// column stage needs four iterations later (metric, 1/g, wind) ride in the same rings.
-// The inputs of row y+PF are requested at the end of iteration y, PF = 2 rows ahead, so their
-// memory latency overlaps a whole iteration of arithmetic: the row's state, correction metric
-// and y-edge wind into one of two rotating buffers, the step-invariant tables and winds
-// straight into their ring slots.
-// (src/discrete_operators.py:18-101, src/flux.py, src/reconstruction_1d.py,
-// src/advection_timestep.py:43)
-template <typename T>
-struct StepIn {     // inputs of iteration y at the lane's column: row y and y-edge y-3
-  T q;                       // row y: Q
-  T gf;                      // row y: metric the correction total multiplies (gfix)
-  T raw;                     // row y: the stored value before the correction total is added
-  T qa;                      // first rows only: Q of row y in the other state buffer
-  T v, vs;                   // y-edge y-3: wind, upwind-side wind
-};
-
-// SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07); the metric treatment follows from it, since
-// the valid pairs are (AVLT or L04, MT-0) and (PL07, MT-PL07) (src/discrete_operators.py:48-82).
-template <typename T, int RECON, bool RK2, int SPLIT, bool WIND = false, bool XPULL = false>
-struct March {
-  static constexpr int split = SPLIT;
-  // rows whose inputs are in flight ahead of the one being processed (a third row of state
-  // measured slower: its extra row offsets cost more than the longer lead saves)
-  static constexpr int PF = 2;
-  static constexpr bool mt0 = SPLIT != 3;
-  // ---- inputs (per wavefront and tracer)
-  const T *Q, *Qa, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
-  // WIND: the winds of the step are formed here from the one-field FP64 basis (WU, WV) and its
-  // coefficient ws at the step's time, as k_wind_avg forms them for RK1, instead of read
-  const double *WU, *WV;
-  double ws;
-  T *Qo;
-  int ld, ldu, M, iend, x, xc, xe, tj, ylast;
-  int am2, am1, ap1, ap2;    // byte addresses of the lanes -2, -1, +1, +2 (mod 64) for lane pulls
-  bool out_lane, need_col;
-  T dtdx, dtdy;
-  double cf;
-  // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6).
-  // The step-invariant row inputs (metric, 1/g, winds) are loaded straight into the slot their
-  // row will hold: with PF = 2 the loads of row y+2 are issued at the end of iteration y, when
-  // that slot's previous row, y-4, has had its last use.  gvr holds gv at y-edges by the same
-  // phase (edge y-2 in slot K % 6), whose previous edge, y-6, was last read at iteration y-2.
-  T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6], gvr[6];
-  T sw[6];                   // FP32: stored values S of the rows (the update is S + flux)
-  T g1prev, g2prev, vprev;
-  T pa_A, pa_B, pa_q6, pb_A, pb_B, pb_q6;   // row y-4 as the left (upwind) cell of edge y-3: Q, Qx
-  double part;
-
-  // Inputs of row y, whose ring slot is SLOT.  SPEC: also read Q from the other state buffer
-  // (Qa), for the rows requested before the step counter has arrived.
-  template <bool SPEC, int SLOT>
-  __device__ __forceinline__ void load(int y, StepIn<T> &r) {
-    static_assert(PF == 2, "the ring slots are reused two rows ahead");
-    const int yc = min(y, M - 1);
-    const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
-    if constexpr (WIND) uw[SLOT] = T(wind_of_basis(ws, ld_at(WU, ou)));
-    else uw[SLOT] = ld_at(U, ou);
-    if constexpr (RK2) usw[SLOT] = ld_at(US, ou);
-    guw[SLOT] = ld_at(gu, ou);
-    gcw[SLOT] = ld_at(gc, o);
-    rgw[SLOT] = ld_at(rgc, o);
-    r.gf = ld_at(gfix, o);
-    const bool reach = need_col && y >= 1 && y < iend + 3;
-    if constexpr (std::is_same<T, float>::value) {
-      // FP32: Q of every lane is read (clamped row and column: rows past ylast are requested
-      // too) and zeroed outside the stencils' reach by a select, with no branch around the load
-      // (measured faster for FP32, slower for FP64, whose registers it raises)
-      const T qv = ld_at(Q, o);
-      r.q = reach ? qv : T(0);
-      if constexpr (SPEC) {
-        const T qa = ld_at(Qa, o);
-        r.qa = reach ? qa : T(0);
-      }
-    } else {
-      r.q = T(0);    // rows outside the stencils' reach
-      r.qa = T(0);
-      if (reach) {
-        r.q = ld_at(Q, (uint32_t)(y * ld + x));
-        if constexpr (SPEC) r.qa = ld_at(Qa, (uint32_t)(y * ld + x));
-      }
-    }
-    const int ec = min(max(y - 3, 0), M - 1);
-    if constexpr (WIND) r.v = T(wind_of_basis(ws, ld_at(WV, (uint32_t)(ec * ld + xc))));
-    else r.v = ld_at(V, (uint32_t)(ec * ld + xc));
-    r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
-    gvr[SLOT] = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
-  }
-
-  // the upwind-side x-wind of the row in a slot (RK1: the wind itself)
-  __device__ __forceinline__ T wind_sign(int slot) const { return RK2 ? usw[slot] : uw[slot]; }
-
-  // every cell the stencils read carries the mass-fixer correction total C: Q = S + C gfix,
-  // gfix being g inside the panel and, at a ghost cell, the weighted metric of the cells its
-  // value was interpolated from (in the solve's precision: C gfix is ~1e-6 of q at most, so
-  // its FP32 rounding is far below that of the sum, which is rounded once).  Lanes and rows
-  // outside the stencils' reach hold 0 + C gfix, which no stored cell reads.
-  // (with no correction total, cf = 0 and q + 0 gfix = q: no branch on the mass fixer)
-  __device__ __forceinline__ void correct(StepIn<T> &r) const {
-    r.raw = r.q;
-    r.q = r.q + T(cf) * r.gf;
-  }
-
-  // Iteration y in phase K.  STAGE: how much of the column stage has begun, fixed by the row's
-  // offset from the item's first row y0 = tj - 3: parabolas of row y-3 from y = tj+2 on
-  // (STAGE >= 1), fluxes through y-edge y-3 from tj+3 on (>= 2), the update of row y-4 from
-  // tj+4 on (3).  The first seven rows are peeled with their stage; every later row is 3, so
-  // the loop carries no per-row test.
-  template <int K, int STAGE>
-  __device__ __forceinline__ void body(const int y, StepIn<T> &in) {
-    constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
-                  s5 = (K + 5) % 6;
-    correct(in);
-    const T gvm1 = gvr[s4], gv0 = gvr[s5], gvp1 = gvr[s0];   // gv at y-edges y-4, y-3, y-2
-    // ---- column stage: parabolas of row y-3, y-edge e = y-3, output row r = y-4
-    if constexpr (STAGE >= 1) {
-      const T gce = gcw[s3];
-      const Parab<T> a = parabola<T, RECON>(qw[s1], qw[s2], qw[s3], qw[s4], qw[s5], gv0, gvp1, gce, mt0);
-      const Parab<T> b = parabola<T, RECON>(xw[s1], xw[s2], xw[s3], xw[s4], xw[s5], gv0, gvp1, gce, mt0);
-      if constexpr (STAGE >= 2) {
-        const T v = in.v;
-        const bool vup = in.vs >= T(0);
-        const T cy = v * dtdy;
-        const T g1 = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, gv0, mt0);     // times dt/dy
-        const T g2 = upwind<T>(vup, pb_A, pb_B, pb_q6, b, cy, gv0, mt0);
-        if constexpr (STAGE >= 3) {
-          const T gcm = gcw[s2];               // row y-4
-          const T dG1 = g1prev - g1;
-          T d1 = T(0), d2 = T(0);
-          if (split != 1) {
-            d1 = gv0 * cy;
-            d2 = gvm1 * (vprev * dtdy);
-          }
-          const T rg = rgw[s2];
-          const T qy = split_update<T>(split, qw[s2], dG1, gcm, rg, d1, d2);
-          const T ur = uw[s2];
-          const bool urp = wind_sign(s2) >= T(0);
-          const T gur = guw[s2];
-          T qy_m2, qy_m1, qy_p1, qy_p2;
-          xwindow<T, XPULL>(qy, am2, am1, ap1, ap2, qy_m2, qy_m1, qy_p1, qy_p2);
-          const Parab<T> c = parabola<T, RECON>(qy_m2, qy_m1, qy, qy_p1, qy_p2, gur, from_right(gur), gcm, mt0);
-          const T f2 = upwind_x<T>(urp, c, ur * dtdx, gur, from_right(gur), mt0);
-          const T dF2 = f2 - from_right(f2);
-          const T dG2 = g2prev - g2;
-          // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
-          const T flx = dF2 + dG2;
-          if (out_lane) {
-            // the correction total stays out of the stored state: S(n+1) = S(n) + flux
-            T out;
-            if constexpr (std::is_same<T, float>::value) out = sw[s2] + flx * rg;
-            else out = (qw[s2] + flx * rg) - T(cf) * gcm;
-            st_at(Qo, (uint32_t)((y - 4) * ld + x), out);
-            part -= (double)flx;               // used with MF-PR only
-          }
-        }
-        g1prev = g1;
-        g2prev = g2;
-        vprev = v;
-      }
-      pa_A = a.rA; pa_B = a.rB; pa_q6 = a.q6;
-      pb_A = b.rA; pb_B = b.rB; pb_q6 = b.q6;
-    }
-    // ---- row stage: row y
-    const T q = in.q, u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
-    const bool up = wind_sign(s0) >= T(0);
-    const T gup1 = from_right(gu0);
-    T q_m2, q_m1, q_p1, q_p2;
-    xwindow<T, XPULL>(q, am2, am1, ap1, ap2, q_m2, q_m1, q_p1, q_p2);
-    const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
-    const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);
-    const T dF1 = f1 - from_right(f1);
-    T c1 = T(0), c2 = T(0);
-    if (split != 1) {
-      c1 = gup1 * (from_right(u) * dtdx);
-      c2 = gu0 * (u * dtdx);
-    }
-    const T qx = split_update<T>(split, q, dF1, gc0, rgw[s0], c1, c2);
-    // row y replaces row y-6 in the rings of computed values (the loaded ones are there already)
-    qw[s0] = q; xw[s0] = qx;
-    if constexpr (std::is_same<T, float>::value) sw[s0] = in.raw;
-    // rows past ylast are read in bounds (clamped rows, masked Q) and never used
-    load<false, (K + PF) % 6>(y + PF, in);
-  }
-
-  // The first PF rows' inputs, requested before the step counter is known: Q from both state
-  // buffers (Q = q0, Qa = q1), the rest from the step-invariant tables and winds.
-  __device__ __forceinline__ void start(StepIn<T> *b) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = gvr[k] = sw[k] = T(0);
-    g1prev = g2prev = vprev = T(0);   // gv is complete from y = tj-1 on; the column stage starts at tj+2
-    pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
-    part = 0.0;
-    // y0 = tj - 3 has phase 0; ylast - y0 = ny + 6 >= PF: every initial row exists
-    load<true, 0>(tj - 3, b[0]);
-    load<true, 1>(tj - 2, b[1]);
-  }
-
-  // The step's input buffer is known: keep the first rows' Q from it and read the rest there.
-  __device__ __forceinline__ void resolve(StepIn<T> *b, bool odd) {
-#pragma unroll
-    for (int k = 0; k < PF; ++k) b[k].q = odd ? b[k].qa : b[k].q;
-    Q = odd ? Qa : Q;
-  }
-
-  __device__ __forceinline__ void run(StepIn<T> *b) {
-    // rows y0 .. y0+6 (ylast - y0 = ny + 6 >= 7: they all exist), the column stage starting
-    int y = tj - 3;
-    body<0, 0>(y, b[0]); body<1, 0>(y + 1, b[1]); body<2, 0>(y + 2, b[0]); body<3, 0>(y + 3, b[1]);
-    body<4, 0>(y + 4, b[0]); body<5, 1>(y + 5, b[1]); body<0, 2>(y + 6, b[0]);
-    // rows y0+7 on: the full step, six rows (phases 1 .. 5, 0) per iteration
-    for (y += 7; y + 5 <= ylast; y += 6) {
-      body<1, 3>(y, b[1]); body<2, 3>(y + 1, b[0]); body<3, 3>(y + 2, b[1]);
-      body<4, 3>(y + 3, b[0]); body<5, 3>(y + 4, b[1]); body<0, 3>(y + 5, b[0]);
-    }
-    if (y <= ylast) body<1, 3>(y, b[1]);
-    if (y + 1 <= ylast) body<2, 3>(y + 1, b[0]);
-    if (y + 2 <= ylast) body<3, 3>(y + 2, b[1]);
-    if (y + 3 <= ylast) body<4, 3>(y + 3, b[0]);
-    if (y + 4 <= ylast) body<5, 3>(y + 4, b[1]);
-  }
-};
-
-template <typename T, int RECON, bool RK2, int SPLIT, bool WIND, bool XPULL>
-__device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
-  const int lane = threadIdx.x;
-  const csadv_solve_desc &D = S.d;
-  const int p = wk.y & 7, tr = wk.y >> 3;    // panel, tracer
-  const int N = D.N;
-  const int ti = NG + wk.z * S.strip_w, tj = NG + wk.w * S.seg_l;
-  const int iend = N + NG;
-  const int nx = min(S.strip_w, iend - ti), ny = min(S.seg_l, iend - tj);
-  const int item = (p * S.segs + wk.w) * S.strips + wk.z;
-  const bool mfpr = D.mf == 3;
-  March<T, RECON, RK2, SPLIT, WIND, XPULL> m;
-  m.ld = D.ld;
-  m.ldu = D.ldu;
-  m.M = S.M;
-  m.iend = iend;
-  m.x = ti - 3 + lane;
-  m.xc = min(m.x, m.M - 1);          // clamped (lanes past the array)
-  m.xe = min(m.x, m.M);
-  m.tj = tj;
-  m.ylast = tj + ny + 3;
-  m.am2 = ((lane + 62) & 63) << 2;
-  m.am1 = ((lane + 63) & 63) << 2;
-  m.ap1 = ((lane + 1) & 63) << 2;
-  m.ap2 = ((lane + 2) & 63) << 2;
-  m.out_lane = lane >= 3 && lane < 3 + nx;
-  m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
-  m.dtdx = T(D.dt / D.dx);
-  m.dtdy = T(D.dt / D.dy);
-  m.U = reinterpret_cast<const T *>(D.uavg) + p * S.pstride_u;
-  m.V = reinterpret_cast<const T *>(D.vavg) + p * S.pstride_v;
-  // RK2 picks the upwind side from separate (un-averaged) winds; RK1 from the winds themselves
-  m.US = RK2 ? reinterpret_cast<const T *>(D.usgn) + p * S.pstride_u : m.U;
-  m.VS = RK2 ? reinterpret_cast<const T *>(D.vsgn) + p * S.pstride_v : m.V;
-  m.gc = reinterpret_cast<const T *>(D.gc);
-  m.gu = reinterpret_cast<const T *>(D.gu);
-  m.gv = reinterpret_cast<const T *>(D.gv);
-  m.rgc = reinterpret_cast<const T *>(D.rgc);
-  m.gfix = reinterpret_cast<const T *>(D.gfix);
-  const int64_t off = tr * S.tstride + p * S.pstride;
-  m.Q = reinterpret_cast<const T *>(D.q0) + off;
-  m.Qa = reinterpret_cast<const T *>(D.q1) + off;
-  StepIn<T> b[March<T, RECON, RK2, SPLIT, WIND, XPULL>::PF];
-  int step;
-  if constexpr (WIND) {
-    // the winds of the first rows need the step's coefficient: the step counter comes first
-    step = D.ctl[2];
-    const int row = min(step, D.max_steps - 1);     // as k_wind_avg (one basis field)
-    m.ws = D.coef[row * 2];
-    m.WU = D.wbu + p * S.pstride_u;
-    m.WV = D.wbv + p * S.pstride_v;
-    m.start(b);
-  } else {
-    m.start(b);
-    // read after the first rows are requested: a wait for any scalar load waits for all of them
-    // (they return out of order), so reading these first would hold back the rows' loads
-    step = D.ctl[2];
-  }
-  const double c0 = mfpr ? D.corr[tr] : 0.0, c1 = mfpr ? D.corr[D.ntr + tr] : 0.0;
-  if (D.stop > 0 && step >= D.stop) return;   // solve finished
-  const bool odd = step & 1;                  // the state of step n lives in q[n & 1]
-  m.resolve(b, odd);
-  m.Qo = reinterpret_cast<T *>(odd ? D.q0 : D.q1) + off;
-  m.cf = odd ? c1 : c0;
-  m.run(b);
-  if (mfpr) {
-    const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
-    if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
-  }
-  if (item == 0 && tr == 0 && lane == 0) {   // hand the step over to the next kernels of the stream
-    D.ctl[1] = step + 1;
-    if (mfpr) D.ctl[3] = 1;
-  }
-}
-
-// Bring every cache line of the kernel-argument descriptor into the scalar cache with loads
-// issued together: an empty asm takes one word of each line and passes v through unchanged, so
-// v exists only once all of them have arrived (one scalar round trip); the field loads at their
-// first use then hit the cache instead of waiting for their lines one after another (the
-// one-solve step kernel's start took four such dependent round trips).  The asm is neither
-// volatile nor a memory clobber, so the loads after it may still be scalar loads.  (For a
-// descriptor in global memory the compiler reads the words with vector loads, which warm no
-// scalar cache: the multi-solve kernel does not use it.)
-__device__ __forceinline__ int touch_solve(const Solve &S, int v) {
-  static_assert(sizeof(Solve) == 424, "one word in each 64-byte line of the descriptor");
-  const char *b = reinterpret_cast<const char *>(&S);
-  int w[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) __builtin_memcpy(&w[k], b + (k < 7 ? 64 * k : 420), 4);
-  asm("" : "=s"(v) : "0"(v), "s"(w[0]), "s"(w[1]), "s"(w[2]), "s"(w[3]), "s"(w[4]), "s"(w[5]), "s"(w[6]), "s"(w[7]));
-  return v;
-}
-
-template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
-__global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves, const int4 *__restrict__ work) {
-  const int4 wk = work[blockIdx.x];
-  if (wk.x < 0) return;              // padding item of the XCD-banded work list
-  step_item<T, RECON, RK2, SPLIT, WIND, false>(solves[wk.x], wk);
-}
-
-// Work item b of a single-solve handle, computed instead of read: the list the host builds
-// (csadv_update) gives workgroup b, which runs on XCD b % kXcds, item b / kXcds of that XCD's
-// band; band x holds the row segments sg with sg * kXcds / segs == x, each segment listing
-// its (panel, strip, tracer) items with the tracer fastest.  Past the band's end: padding.
-__device__ __forceinline__ bool single_item(const Solve &S, int b, int4 &wk) {
-  const int x = b % kXcds, i = b / kXcds;
-  const int lo = (x * S.segs + kXcds - 1) / kXcds, hi = ((x + 1) * S.segs + kXcds - 1) / kXcds;
-  const int ntr = S.d.ntr, row = S.strips * ntr, per = 6 * row;
-  if (i >= (hi - lo) * per) return false;
-  const int r = i % per, r2 = r % row;
-  wk = make_int4(0, r / row + 8 * (r2 % ntr), r2 / ntr, lo + i / per);
-  return true;
-}
-
-// A handle of one solve: the descriptor travels in the kernel arguments and the work item is
-// computed, so the first loads of a wavefront depend on nothing but the arguments.
-// FP32 takes its x windows through the crossbar (measured 3% faster at N=768 with 4 tracers);
-// FP64 and the multi-solve kernel keep the DPP shifts (measured 0.5-4% slower with the crossbar).
-template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
-__global__ __launch_bounds__(WAVE) void k_step1(const Solve one) {
-  int4 wk;
-  if (!single_item(one, touch_solve(one, blockIdx.x), wk)) return;
-  step_item<T, RECON, RK2, SPLIT, WIND, std::is_same<T, float>::value>(one, wk);
 }
 
 // Fold the correction totals into the current buffer (interior only); k_clear_corr then
@@ -848,7 +492,7 @@ __global__ void k_wind_avg(Solve *solves, int all) {
   const int p = blockIdx.y;
   const int i0 = NG, iend = N + NG;
   if (!all && !D.unsteady) return;                                  // steady: set at init
-  if (D.stop > 0 && D.ctl[1] >= D.stop) return;                    // solve finished
+  if ((D.stop > 0 && D.ctl[1] >= D.stop) || D.ctl[0]) return;       // solve finished or halted
   const int row = D.unsteady ? min(D.ctl[1], D.max_steps - 1) : 0;   // steps completed
   const double *s1 = D.coef + (int64_t)row * 2 * K;   // coefficients at t_{n-1}
   const double *s0 = s1 + K;                          // at t_{n-2}
@@ -908,210 +552,190 @@ __host__ __device__ __forceinline__ void kadd(KSum &a, double x) {
   if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x; else a.c += (x - t) + a.s;
   a.s = t;
 }
+__device__ __forceinline__ void kmerge(KSum &a, const KSum &b) {
+  kadd(a, b.s);
+  a.c += b.c;
+}
 
+// Butterfly reductions over the 64 lanes of a wavefront (cross-lane shuffles, no LDS): lane 0
+// ends with the wavefront's total, combined in a fixed order.
+__device__ __forceinline__ void wave_ksum(KSum &a) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const KSum b{__shfl_xor(a.s, off, WAVE), __shfl_xor(a.c, off, WAVE)};
+    kmerge(a, b);
+  }
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
+// The 8 words of a diagnostics partial: mass (sum, compensation), sum |e| (sum, comp.),
+// sum e^2 (sum, comp.), max |e| and the count of NaN values of e (fmax passes over NaN).
+// Wavefront totals by shuffles, then thread 0 merges the block's wavefronts in index order:
+// the result does not depend on scheduling.
+__device__ __forceinline__ void diag_block(KSum m, KSum a1, KSum a2, double mx, double nan, double *o) {
+  __shared__ double red[BLOCK / WAVE][kDiagWords];
+  wave_ksum(m);
+  wave_ksum(a1);
+  wave_ksum(a2);
+  mx = wave_max(mx);
+  nan = wave_sum(nan);
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) {
+    red[w][0] = m.s; red[w][1] = m.c; red[w][2] = a1.s; red[w][3] = a1.c;
+    red[w][4] = a2.s; red[w][5] = a2.c; red[w][6] = mx; red[w][7] = nan;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    KSum M{red[0][0], red[0][1]}, A1{red[0][2], red[0][3]}, A2{red[0][4], red[0][5]};
+    double X = red[0][6], B = red[0][7];
+    for (int k = 1; k < BLOCK / WAVE; ++k) {
+      kmerge(M, KSum{red[k][0], red[k][1]});
+      kmerge(A1, KSum{red[k][2], red[k][3]});
+      kmerge(A2, KSum{red[k][4], red[k][5]});
+      X = fmax(X, red[k][6]);
+      B += red[k][7];
+    }
+    o[0] = M.s; o[1] = M.c; o[2] = A1.s; o[3] = A1.c; o[4] = A2.s; o[5] = A2.c; o[6] = X; o[7] = B;
+  }
+}
+
+// Diagnostics of tracers tracer0 .. tracer0 + gridDim.y - 1 of one solve (grid: kDiagBlocks x
+// tracers): per block the compensated FP64 sums of the mass Q g dx dy (src/diagnostics.py:14-26)
+// and of |e| and e^2, and the max |e| (src/errors.py:99-113), with e = qexact - Q when the exact
+// field is given ([tracers][6][N][N] FP64) and e = Q otherwise.  Q = S + C g (the pending
+// mass-fixer total), read from the solve's current buffer.
 template <typename T>
-__global__ void k_mass(const Solve *solves, int solve, int tracer, double *partial) {
+__global__ __launch_bounds__(BLOCK) void k_diag(const Solve *solves, int solve, int tracer0, const double *qex,
+                                                double *part) {
   const Solve &S = solves[solve];
   const csadv_solve_desc &D = S.d;
+  const int tr = tracer0 + blockIdx.y;
   const int N = D.N, ld = D.ld;
-  const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
-  const double cf = D.corr ? D.corr[(D.ctl[1] & 1) * D.ntr + tracer] : 0.0;
+  const int cur = D.ctl[1] & 1;
+  const T *Q = reinterpret_cast<const T *>(cur ? D.q1 : D.q0) + tr * S.tstride;
+  const double cf = (D.corr && D.mf == 3) ? D.corr[cur * D.ntr + tr] : 0.0;
   const double dxdy = D.dx * D.dy;
-  KSum acc{0.0, 0.0};
-  const int64_t tot = (int64_t)6 * N * N;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
-    const int p = (int)(idx / ((int64_t)N * N));
-    const int rem = (int)(idx - (int64_t)p * N * N);
+  const int64_t cells = (int64_t)6 * N * N, nn = (int64_t)N * N;
+  const double *qe = qex ? qex + (int64_t)blockIdx.y * cells : nullptr;
+  KSum m{0.0, 0.0}, a1{0.0, 0.0}, a2{0.0, 0.0};
+  double mx = 0.0, nan = 0.0;
+  for (int64_t idx = blockIdx.x * (int64_t)BLOCK + threadIdx.x; idx < cells; idx += (int64_t)gridDim.x * BLOCK) {
+    const int p = (int)(idx / nn);
+    const int rem = (int)(idx - p * nn);
     const int j = rem / N + NG, i = rem % N + NG;
     const double g = D.gc64[(int64_t)j * ld + i];
     const double q = (double)Q[p * S.pstride + (int64_t)j * ld + i] + cf * g;
-    kadd(acc, q * g * dxdy);
-  }
-  __shared__ double ss[BLOCK], sc[BLOCK];
-  ss[threadIdx.x] = acc.s;
-  sc[threadIdx.x] = acc.c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    KSum b{0.0, 0.0};
-    for (int t = 0; t < (int)blockDim.x; ++t) { kadd(b, ss[t]); kadd(b, sc[t]); }
-    partial[2 * blockIdx.x] = b.s;
-    partial[2 * blockIdx.x + 1] = b.c;
-  }
-}
-
-template <typename T>
-__global__ void k_norms(const Solve *solves, int solve, int tracer, const double *qex, double *partial) {
-  const Solve &S = solves[solve];
-  const csadv_solve_desc &D = S.d;
-  const int N = D.N, ld = D.ld;
-  const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
-  const double cf = D.corr ? D.corr[(D.ctl[1] & 1) * D.ntr + tracer] : 0.0;
-  KSum s1{0.0, 0.0}, s2{0.0, 0.0};
-  double mx = 0.0;
-  const int64_t tot = (int64_t)6 * N * N;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
-    const int p = (int)(idx / ((int64_t)N * N));
-    const int rem = (int)(idx - (int64_t)p * N * N);
-    const int j = rem / N + NG, i = rem % N + NG;
-    const double q = (double)Q[p * S.pstride + (int64_t)j * ld + i] + cf * D.gc64[(int64_t)j * ld + i];
-    const double e = fabs(qex[idx] - q);
-    kadd(s1, e);
-    kadd(s2, e * e);
+    kadd(m, q * g * dxdy);
+    const double e = fabs(qe ? qe[idx] - q : q);
+    kadd(a1, e);
+    kadd(a2, e * e);
     mx = fmax(mx, e);
+    nan += e != e ? 1.0 : 0.0;
   }
-  __shared__ double a[BLOCK], b[BLOCK], c[BLOCK], d[BLOCK], m[BLOCK];
-  a[threadIdx.x] = s1.s; b[threadIdx.x] = s1.c; c[threadIdx.x] = s2.s; d[threadIdx.x] = s2.c; m[threadIdx.x] = mx;
+  diag_block(m, a1, a2, mx, nan, part + ((int64_t)blockIdx.y * kDiagBlocks + blockIdx.x) * kDiagWords);
+}
+
+// The tracers' totals from the block partials (one block per tracer, thread t takes partial t):
+// out[4 k .. 4 k + 3] = mass, Linf, L1, L2 (L1, L2 as unweighted means over the 6 N^2 cells;
+// Linf is NaN when any e is, as numpy's amax gives).  limit > 0 and Linf > limit -- or a NaN --
+// halt the solve (ctl[0] = 1): its step kernels leave it untouched from then on, as the
+// reference stops a run whose error exceeds 100 (src/output.py:44-47).
+__global__ __launch_bounds__(BLOCK) void k_diag_final(Solve *solves, int solve, const double *part, double limit,
+                                                      double *out) {
+  static_assert(kDiagBlocks == BLOCK, "one partial per thread");
+  csadv_solve_desc &D = solves[solve].d;
+  const double *o = part + ((int64_t)blockIdx.x * kDiagBlocks + threadIdx.x) * kDiagWords;
+  __shared__ double tot[kDiagWords];
+  diag_block(KSum{o[0], o[1]}, KSum{o[2], o[3]}, KSum{o[4], o[5]}, o[6], o[7], tot);
   __syncthreads();
   if (threadIdx.x == 0) {
-    KSum t1{0.0, 0.0}, t2{0.0, 0.0};
-    double mm = 0.0;
-    for (int t = 0; t < (int)blockDim.x; ++t) { kadd(t1, a[t]); kadd(t1, b[t]); kadd(t2, c[t]); kadd(t2, d[t]); mm = fmax(mm, m[t]); }
-    partial[5 * blockIdx.x + 0] = t1.s; partial[5 * blockIdx.x + 1] = t1.c;
-    partial[5 * blockIdx.x + 2] = t2.s; partial[5 * blockIdx.x + 3] = t2.c;
-    partial[5 * blockIdx.x + 4] = mm;
+    const double n = 6.0 * D.N * D.N;
+    const double linf = tot[7] > 0.0 ? __builtin_nan("") : tot[6];
+    double *r = out + 4 * blockIdx.x;
+    r[0] = tot[0] + tot[1];
+    r[1] = linf;
+    r[2] = (tot[2] + tot[3]) / n;
+    r[3] = sqrt((tot[4] + tot[5]) / n);
+    if (limit > 0.0 && (linf > limit || tot[7] > 0.0)) D.ctl[0] = 1;
   }
 }
 
-#include "csadv_general.hip"
+// general-step kernels that are not templates (the templates: csadv_general.hpp)
+namespace gp {
 
-// ------------------------------------------------------------------ host side
-struct csadv_s {
-  std::vector<csadv_solve_desc> desc;
-  std::vector<Solve> host;
-  Solve *dev = nullptr;           // device descriptors (caller-visible scratch is not used)
-  int4 *work = nullptr;           // step work items
-  FieldSet *qsets = nullptr;      // ghost-fill sets for the tracers
-  FieldSet *wsets = nullptr;      // ghost-fill sets for the lat-lon centre winds
-  int nwork = 0;
-  int tile = 0;
-  int dtype = 0;
-  int recon = 3;
-  bool rk2 = false;               // dp = 2 (separate upwind-side winds)
-  int split = 1;                  // splitting of every solve (its metric treatment follows)
-  bool general = false;           // ET-S72 / ET-PL07 / MF-AF: the general step
-  bool wind1 = false;             // every solve RK1 with a one-field time-dependent wind: the
-                                  // step kernel forms its winds (no per-step k_wind_avg)
-  double *red = nullptr;          // small reduction scratch (mass / norms)
-  std::vector<int> done;          // host copy of the steps completed per solve (ctl[1])
-  std::vector<hipEvent_t> ev;     // profiling event pairs (start, end) around the step kernel
-  int ev_used = 0;
-  int graph_steps = 0;            // steps per captured graph (0 = plain launches)
-  hipStream_t cap = nullptr;      // private stream used only for capture
-  hipGraphExec_t gexec = nullptr; // instantiated graph of graph_steps steps
-  std::string err;
-};
-
-namespace {
-
-// tile id -> rows per wavefront segment (the strip width is always the widest that fits)
-constexpr int kSegRows[] = {8, 4, 16, 32, 6, 12, 11, 5, 10, 14, 17, 23};
-constexpr int kNumTiles = sizeof(kSegRows) / sizeof(kSegRows[0]);
-
-void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs) {
-  strips = (N + STRIP_MAX - 1) / STRIP_MAX;
-  strip_w = (N + strips - 1) / strips;          // balanced strips of at most 58 columns
-  seg_l = std::min(kSegRows[tile], N);
-  segs = (N + seg_l - 1) / seg_l;
+// ET-S72 / ET-PL07 wind halo for RK2 (src/edges_treatment.py:305-347), on every wind basis
+// field: the ghost edge one outside each panel side takes the neighbour's normal wind at its
+// edge one inside the shared edge, times the axis sign across the edge.
+__global__ void k_gp_wind_rk2_copy(const Solve *solves, int s) {
+  const Solve &S = solves[s];
+  const csadv_solve_desc &D = S.d;
+  const int ld = D.ld, ldu = D.ldu, i0 = NG, iend = D.N + NG;
+  const int64_t bu = 6 * S.pstride_u, bv = 6 * S.pstride_v;
+  const unsigned N = D.N, total = (unsigned)D.nbasis * 24u * N;
+  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const unsigned u = t / N;                        // u = (b * 6 + p) * 4 + side
+    const int k = (int)(t % N) + i0;
+    const int side = u & 3, p = (u >> 2) % 6, b = (u >> 2) / 6;
+    double *U = D.wbu + b * bu, *V = D.wbv + b * bv;
+    const int32_t *o = D.orient + (p * 4 + side) * 7;
+    int q, i, j, sq, sign;
+    halo_src(o, 0, k, q, i, j);
+    across(o, side, sq, sign);
+    // the neighbour's normal wind one edge inside the shared edge, x-edges (E/W) or y-edges (N/S);
+    // both addresses are formed and one is selected
+    const int ne = (sq == 0 || sq == 2) ? iend - 1 : i0 + 1;
+    const double *nu = U + q * S.pstride_u + (int64_t)j * ldu + ne;
+    const double *nv = V + q * S.pstride_v + (int64_t)ne * ld + i;
+    const double nb = *(sq < 2 ? nu : nv);
+    const double val = sign > 0 ? nb : -nb;
+    // our ghost edge one outside the side
+    const int te = (side == 0 || side == 2) ? iend + 1 : i0 - 1;
+    double *tu = U + p * S.pstride_u + (int64_t)k * ldu + te;
+    double *tv = V + p * S.pstride_v + (int64_t)te * ld + k;
+    *(side < 2 ? tu : tv) = val;
+  }
 }
-constexpr int kRedBlocks = 256;
+
+// the step is complete for every tracer: advance the step counter
+__global__ void k_gp_advance(const Solve *solves, int s) {
+  const csadv_solve_desc &D = solves[s].d;
+  if ((D.stop > 0 && D.ctl[1] >= D.stop) || D.ctl[0]) return;     // finished or halted
+  if (threadIdx.x == 0) D.ctl[1] = D.ctl[1] + 1;
+}
+
+}  // namespace gp
+
+void launch_gp_advance(csadv_t *h, hipStream_t st, int s) {
+  hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);
+}
+
 
 int fail(csadv_t *h, int code, const std::string &msg) {
   if (h) h->err = msg;
   return code;
 }
 
-#define HIPCHK(h, call)                                                              \
-  do {                                                                               \
-    hipError_t e_ = (call);                                                          \
-    if (e_ != hipSuccess)                                                            \
-      return fail((h), CSADV_E_HIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
-  } while (0)
-
-int validate(const csadv_solve_desc &d, std::string &msg) {
-  if (d.N < 8 || d.N > 8192) { msg = "N must be in [8, 8192]"; return CSADV_E_RANGE; }
-  if (d.ntr < 1 || d.ntr > 64) { msg = "ntr must be in [1, 64]"; return CSADV_E_RANGE; }
-  if (d.dtype != CSADV_F64 && d.dtype != CSADV_F32) { msg = "dtype"; return CSADV_E_ARG; }
-  const int M = d.N + 2 * NG;
-  if (d.ld < M || d.ldu < M + 1) { msg = "row strides too small"; return CSADV_E_ARG; }
-  if (d.recon < 1 || d.recon > 4) { msg = "recon must be 1..4"; return CSADV_E_ARG; }
-  if (d.dp != 1 && d.dp != 2) { msg = "dp must be 1 or 2"; return CSADV_E_ARG; }
-  if (!((d.split == 1 || d.split == 2) && d.mt == 1) && !(d.split == 3 && d.mt == 2)) {
-    msg = "invalid (opsplit, mt) pair: valid pairs are (1|2, 1) and (3, 2)";
-    return CSADV_E_ARG;
+int ghost_fill(csadv_t *h, hipStream_t st) {
+  const int ns = (int)h->desc.size();
+  if (h->dtype == CSADV_F64) {
+    hipLaunchKernelGGL(k_ghost_edges<double>, dim3(96, ns), dim3(BLOCK), 0, st, h->qsets);
+    hipLaunchKernelGGL(k_ghost_corners<double>, dim3(6, ns), dim3(64), 0, st, h->qsets);
+  } else {
+    hipLaunchKernelGGL(k_ghost_edges<float>, dim3(96, ns), dim3(BLOCK), 0, st, h->qsets);
+    hipLaunchKernelGGL(k_ghost_corners<float>, dim3(6, ns), dim3(64), 0, st, h->qsets);
   }
-  if (d.et < 1 || d.et > 3) { msg = "et must be 1 (ET-S72), 2 (ET-PL07) or 3 (ET-DG)"; return CSADV_E_ARG; }
-  if (d.mf < 1 || d.mf > 3) { msg = "mf must be 1 (none), 2 (MF-AF) or 3 (MF-PR)"; return CSADV_E_ARG; }
-  if ((d.et != 3 || d.mf == 2) && !d.work) {
-    msg = "ET-S72, ET-PL07 and MF-AF run the general step, which needs desc.work (csadv_work_bytes)";
-    return CSADV_E_ARG;
-  }
-  if ((d.et != 3 || d.mf == 2) && (d.ld != (M + 3) / 4 * 4 || d.ldu != (M + 4) / 4 * 4)) {
-    msg = "the general step expects ld = round_up(N+8, 4) and ldu = round_up(N+9, 4) (the csadv_work_bytes layout)";
-    return CSADV_E_ARG;
-  }
-  if (d.nbasis < 1 || d.nbasis > 8) { msg = "nbasis must be 1..8"; return CSADV_E_RANGE; }
-  if (d.unsteady && d.max_steps < 1) { msg = "unsteady winds need a coefficient table"; return CSADV_E_ARG; }
-  if (!d.q0 || !d.q1 || !d.uavg || !d.vavg || !d.gc || !d.gu || !d.gv || !d.rgc || !d.gc64 || !d.lagw ||
-      !d.lagk || !d.orient || !d.wbu || !d.wbv || !d.coef || !d.ctl) {
-    msg = "null device pointer in descriptor";
-    return CSADV_E_ARG;
-  }
-  if (d.dp == 2 && (!d.usgn || !d.vsgn)) { msg = "dp=2 needs the usgn and vsgn wind arrays"; return CSADV_E_ARG; }
-  if (d.mf == 3 && (!d.corr || !d.partials)) { msg = "MF-PR needs corr and partials"; return CSADV_E_ARG; }
-  if (d.et == 3 && d.mf != 2 && !d.gfix) { msg = "the fused step (ET-DG) needs gfix"; return CSADV_E_ARG; }
-  if (d.n_strip != 6 * 4 * 3 * d.N || d.n_corner != 6 * 2 * 3 * 6 || !d.strip_tgt || !d.strip_src || !d.strip_w ||
-      !d.corner_tgt || !d.corner_src || !d.corner_w) {
-    msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";
-    return CSADV_E_ARG;
-  }
-  if (((uintptr_t)d.strip_src | (uintptr_t)d.corner_src | (uintptr_t)d.strip_w | (uintptr_t)d.corner_w) % 16 != 0) {
-    msg = "strip_src, corner_src, strip_w and corner_w must be 16-byte aligned";
-    return CSADV_E_ARG;
-  }
-  if (d.stop < 0) { msg = "stop must be >= 0"; return CSADV_E_RANGE; }
-  if (d.tile < -1 || d.tile >= kNumTiles) { msg = "tile must be -1 or a valid tile id"; return CSADV_E_RANGE; }
-  if (!(d.dt > 0.0) || !(d.dx > 0.0) || !(d.dy > 0.0)) { msg = "dt, dx, dy must be positive"; return CSADV_E_ARG; }
+  HIPCHK(h, hipGetLastError());
   return CSADV_OK;
 }
 
-Solve derive(const csadv_solve_desc &d, int tile) {
-  Solve s{};
-  s.d = d;
-  s.M = d.N + 2 * NG;
-  tiling(d.N, d.tile >= 0 ? d.tile : tile, s.strip_w, s.strips, s.seg_l, s.segs);
-  s.items = s.strips * s.segs;
-  s.pstride = (int64_t)s.M * d.ld;
-  s.tstride = 6 * s.pstride;
-  s.pstride_u = (int64_t)s.M * d.ldu;
-  s.pstride_v = (int64_t)(s.M + 1) * d.ld;
-  s.part_per_tracer = 6 * s.items;
-  return s;
-}
-
-template <typename T, int R, int SP>
-hipError_t launch_step_s(csadv_t *h, hipStream_t st) {
-  const bool one = h->host.size() == 1;
-  if (h->wind1) {                    // RK1 only
-    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP, true>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
-    else hipLaunchKernelGGL((k_step<T, R, false, SP, true>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
-    return hipGetLastError();
-  }
-  if (h->rk2) {
-    if (one) hipLaunchKernelGGL((k_step1<T, R, true, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
-    else hipLaunchKernelGGL((k_step<T, R, true, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
-  } else {
-    if (one) hipLaunchKernelGGL((k_step1<T, R, false, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->host[0]);
-    else hipLaunchKernelGGL((k_step<T, R, false, SP, false>), dim3(h->nwork), dim3(WAVE), 0, st, h->dev, h->work);
-  }
-  return hipGetLastError();
-}
-
-template <typename T, int R>
-hipError_t launch_step_r(csadv_t *h, hipStream_t st) {
-  if (h->split == 2) return launch_step_s<T, R, 2>(h, st);
-  if (h->split == 3) return launch_step_s<T, R, 3>(h, st);
-  return launch_step_s<T, R, 1>(h, st);
-}
+namespace {
 
 template <typename T>
 hipError_t launch_step_dt(csadv_t *h, hipStream_t st) {
@@ -1122,8 +746,6 @@ hipError_t launch_step_dt(csadv_t *h, hipStream_t st) {
     default: return launch_step_r<T, 3>(h, st);
   }
 }
-
-int gp_blocks(int64_t total) { return (int)std::min<int64_t>(4096, (total + BLOCK - 1) / BLOCK); }
 
 void drop_graph(csadv_t *h) {
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
@@ -1145,19 +767,6 @@ int step_ghosts(csadv_t *h, hipStream_t st) {
   } else {
     if (one) hipLaunchKernelGGL(k_ghost1<float>, grid, dim3(BLOCK), 0, st, h->host[0]);
     else hipLaunchKernelGGL(k_ghost<float>, grid, dim3(BLOCK), 0, st, h->dev);
-  }
-  HIPCHK(h, hipGetLastError());
-  return CSADV_OK;
-}
-
-int ghost_fill(csadv_t *h, hipStream_t st) {
-  const int ns = (int)h->desc.size();
-  if (h->dtype == CSADV_F64) {
-    hipLaunchKernelGGL(k_ghost_edges<double>, dim3(96, ns), dim3(BLOCK), 0, st, h->qsets);
-    hipLaunchKernelGGL(k_ghost_corners<double>, dim3(6, ns), dim3(64), 0, st, h->qsets);
-  } else {
-    hipLaunchKernelGGL(k_ghost_edges<float>, dim3(96, ns), dim3(BLOCK), 0, st, h->qsets);
-    hipLaunchKernelGGL(k_ghost_corners<float>, dim3(6, ns), dim3(64), 0, st, h->qsets);
   }
   HIPCHK(h, hipGetLastError());
   return CSADV_OK;
@@ -1185,27 +794,11 @@ extern "C" {
 
 int csadv_abi_version(void) { return CSADV_ABI_VERSION; }
 
-int csadv_partials_per_tracer(int32_t N, int32_t tile) {
-  if (tile < 0 || tile >= kNumTiles || N < 1) return CSADV_E_ARG;
-  int sw, ns, sl, nsg;
-  tiling(N, tile, sw, ns, sl, nsg);
-  return 6 * ns * nsg;
-}
+int csadv_partials_per_tracer(int32_t N, int32_t tile) { return partials_per_tracer(N, tile); }
 
-int csadv_work_bytes(int32_t N, int32_t dtype, int64_t *bytes) {
-  if (!bytes || N < 8 || N > 8192 || (dtype != CSADV_F64 && dtype != CSADV_F32)) return CSADV_E_ARG;
-  const int M = N + 2 * NG;
-  const int ld = (M + 3) / 4 * 4, ldu = (M + 4) / 4 * 4;
-  const gp::Work w = gp::layout(N, ld, ldu);
-  const int64_t s = dtype == CSADV_F64 ? 8 : 4;
-  *bytes = w.total * s + (int64_t)gp::kRedGen * 8;
-  return CSADV_OK;
-}
+int csadv_work_bytes(int32_t N, int32_t dtype, int64_t *bytes) { return work_bytes(N, dtype, bytes); }
 
-int csadv_tile_rows(int32_t tile) {
-  if (tile < 0 || tile >= kNumTiles) return CSADV_E_ARG;
-  return kSegRows[tile];
-}
+int csadv_tile_rows(int32_t tile) { return tile_rows(tile); }
 
 int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, csadv_t **out) {
   if (!out) return CSADV_E_ARG;
@@ -1218,16 +811,11 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
   h->recon = solves[0].recon;
   h->rk2 = solves[0].dp == 2;
   h->split = solves[0].split;
-  h->general = solves[0].et != 3 || solves[0].mf == 2;
+  h->general = is_general(solves[0]);
+  int maxtr = 1;
   for (int s = 0; s < nsolve; ++s) {
     std::string msg;
-    int rc = validate(solves[s], msg);
-    if (rc == CSADV_OK && (solves[s].dtype != h->dtype || solves[s].recon != h->recon || solves[s].split != h->split ||
-                           solves[s].dp != solves[0].dp ||
-                           (solves[s].et != 3 || solves[s].mf == 2) != h->general)) {
-      rc = CSADV_E_ARG;
-      msg = "all solves of one handle share dtype, recon, split, dp and the step kind (fused or general)";
-    }
+    const int rc = validate(solves[s], msg);
     if (rc != CSADV_OK) {
       // the handle is returned so that csadv_last_error() can explain the failure
       h->err = "solve " + std::to_string(s) + ": " + msg;
@@ -1235,14 +823,19 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
       return rc;
     }
     h->desc.push_back(solves[s]);
+    maxtr = std::max(maxtr, solves[s].ntr);
   }
-  h->done.assign(nsolve, 0);
+  std::string msg;
+  const int rc = check_compatible(solves, nsolve, msg);
   *out = h;
+  if (rc != CSADV_OK) return fail(h, rc, msg);
+  h->done.assign(nsolve, 0);
+  h->red_tracers = maxtr;
   hipError_t e;
   if ((e = hipMalloc(&h->dev, sizeof(Solve) * nsolve)) != hipSuccess ||
       (e = hipMalloc(&h->qsets, sizeof(FieldSet) * nsolve)) != hipSuccess ||
       (e = hipMalloc(&h->wsets, sizeof(FieldSet) * nsolve)) != hipSuccess ||
-      (e = hipMalloc(&h->red, sizeof(double) * 5 * kRedBlocks + 64)) != hipSuccess)
+      (e = hipMalloc(&h->red, sizeof(double) * diag_scratch_words(maxtr))) != hipSuccess)
     return fail(h, CSADV_E_HIP, std::string("descriptor allocation: ") + hipGetErrorString(e));
   return csadv_update(h, solves, nsolve, nullptr);
 }
@@ -1253,24 +846,18 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
   hipStream_t st = (hipStream_t)stream;
   drop_graph(h);   // launch shapes may change with the descriptors
   h->host.clear();
-  std::vector<std::vector<int4>> band(kXcds);
   std::vector<FieldSet> sets;
   for (int s = 0; s < nsolve; ++s) {
     std::string msg;
     int rc = validate(solves[s], msg);
+    if (rc == CSADV_OK && solves[s].ntr > h->red_tracers) {
+      rc = CSADV_E_RANGE;
+      msg = "ntr grew past the handle's diagnostics scratch (create a new handle)";
+    }
     if (rc != CSADV_OK) return fail(h, rc, "solve " + std::to_string(s) + ": " + msg);
     h->desc[s] = solves[s];
-    Solve S = derive(solves[s], h->tile);
+    const Solve S = derive(solves[s], h->tile);
     h->host.push_back(S);
-    // segment rows are cut into kXcds contiguous bands, one per XCD: the band's halo rows and
-    // its share of the panel-invariant metric then stay in that XCD's L2 for all six panels
-    // (the tracers of one segment are neighbours in a band, so they share its wind and
-    // metric rows in L2; the work item packs panel and tracer as p + 8 * tracer)
-    for (int sg = 0; sg < S.segs; ++sg)
-      for (int p = 0; p < 6; ++p)
-        for (int sx = 0; sx < S.strips; ++sx)
-          for (int tr = 0; tr < solves[s].ntr; ++tr)
-            band[sg * kXcds / S.segs].push_back(make_int4(s, p + 8 * tr, sx, sg));
     FieldSet F{};
     F.base[0] = solves[s].q0;
     F.base[1] = solves[s].q1;
@@ -1288,13 +875,14 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
     F.orient = solves[s].orient;
     sets.push_back(F);
   }
-  // workgroup b runs on XCD b % kXcds: interleave the bands, padding them to equal length
-  size_t blen = 0;
-  for (auto &bd : band) blen = std::max(blen, bd.size());
-  std::vector<int4> work;
-  work.reserve(blen * kXcds);
-  for (size_t i = 0; i < blen; ++i)
-    for (int x = 0; x < kXcds; ++x) work.push_back(i < band[x].size() ? band[x][i] : make_int4(-1, 0, 0, 0));
+  {
+    std::string msg;
+    const int rc = check_compatible(solves, nsolve, msg);
+    if (rc != CSADV_OK) return fail(h, rc, msg);
+  }
+  // workgroup b runs on XCD b % kXcds (the list interleaves one band of row segments per XCD)
+  static_assert(sizeof(Item) == sizeof(int4), "work items are read as int4");
+  const std::vector<Item> work = work_list(h->host);
   if ((int)work.size() != h->nwork) {
     if (h->work) (void)hipFree(h->work);
     h->work = nullptr;
@@ -1378,99 +966,33 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
 
 namespace {
 
-// ET-PL07 edge values: reconstruction of the cells next to the panel sides with the
-// extrapolation, cube-edge averages, their write-back with the ghost parabolas
-template <typename T, int R>
-void gp_edges(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
-  const csadv_solve_desc &d = h->desc[s];
-  const int64_t sides = (int64_t)6 * 4 * d.N;
-  hipLaunchKernelGGL((gp::k_gp_extrap<T, R>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
-                     fy_field);
-  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
-  hipLaunchKernelGGL((gp::k_gp_edge_ghosts<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
-}
+// Device outputs of csadv_mass / csadv_norms: after the partials of the handle's largest solve.
+double *diag_out(csadv_t *h) { return h->red + (int64_t)h->red_tracers * kDiagBlocks * kDiagWords; }
 
-template <typename T>
-void gp_copy_halo(csadv_t *h, hipStream_t st, int s, int tr, int mode) {
-  const int M = h->desc[s].N + 2 * NG;
-  hipLaunchKernelGGL((gp::k_gp_copy_gather<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
-                     mode);
-  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
-                     mode);
-}
-
-// One step of every solve by the general step (src/discrete_operators.py:18-101 stage by stage).
-template <typename T, int R>
-int enqueue_general_r(csadv_t *h, hipStream_t st) {
-  const int ns = (int)h->desc.size();
-  bool any_dg = false;
-  for (auto &d : h->desc) any_dg = any_dg || d.et == 3;
-  if (any_dg) {                  // ET-DG solves: the full duo-grid halo of Q
-    const int rc = ghost_fill(h, st);
-    if (rc) return rc;
-  }
-  for (int s = 0; s < ns; ++s) {
-    const csadv_solve_desc &d = h->desc[s];
-    const int N = d.N, M = d.N + 2 * NG;
-    const int64_t edges = (int64_t)6 * M * (N + 1), cells = (int64_t)6 * M * M;
-    for (int tr = 0; tr < d.ntr; ++tr) {
-      // ET-PL07 rewrites edge values next to the panel sides between the reconstruction and
-      // the flux, which reads those from the arrays; every other edge value the flux kernel
-      // reconstructs itself
-      const bool fuse = d.et != 2;
-      // x- and y-edge fluxes in one launch: gp_blocks(edges / 2) blocks per direction, two
-      // edge items per lane
-      auto flux = [&](int fx_field, int fy_field) {
-        const int nbx = gp_blocks((edges + 1) / 2);
-        if (fuse)
-          hipLaunchKernelGGL((gp::k_gp_flux<T, R, true>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
-                             fy_field, nbx);
-        else
-          hipLaunchKernelGGL((gp::k_gp_flux<T, R, false>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
-                             fy_field, nbx);
-      };
-      if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 0);
-      if (!fuse) {
-        gp_edges<T, R>(h, st, s, tr, 0, 0);
-      }
-      flux(0, 0);
-      hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev, s, tr);
-      if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
-      // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
-      if (!fuse) {
-        gp_edges<T, R>(h, st, s, tr, 2, 1);
-      }
-      flux(2, 1);
-      if (d.mf == 2) {           // MF-AF: gather the cube-edge averages; k_gp_div_update reads them
-        const int64_t sides = (int64_t)6 * 4 * N;
-        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
-      }
-      if (d.mf == 3) {           // MF-PR: the global sum of div g sits between div and the update
-        hipLaunchKernelGGL((gp::k_gp_div<T>), dim3(gp::kRedGen), dim3(BLOCK), 0, st, h->dev, s, tr);
-        hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
-      } else {
-        if (d.mf == 2)
-          hipLaunchKernelGGL((gp::k_gp_div_update<T, true>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev,
-                             s, tr);
-        else
-          hipLaunchKernelGGL((gp::k_gp_div_update<T, false>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev,
-                             s, tr);
-      }
-    }
-    hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);
-  }
-  HIPCHK(h, hipGetLastError());
+int diag_args(csadv_t *h, int solve, int tracer0, int ntracers, double limit, const double *out) {
+  if (solve < 0 || solve >= (int)h->desc.size()) return fail(h, CSADV_E_ARG, "diagnostics: bad solve index");
+  if (tracer0 < 0 || ntracers < 1 || tracer0 + ntracers > h->desc[solve].ntr)
+    return fail(h, CSADV_E_ARG, "diagnostics: tracers out of range of the solve");
+  if (!out) return fail(h, CSADV_E_ARG, "diagnostics: null output");
+  if (!(limit >= 0.0)) return fail(h, CSADV_E_ARG, "diagnostics: blow-up limit must be >= 0 (0 = none)");
   return CSADV_OK;
 }
 
-template <typename T>
-int enqueue_general_t(csadv_t *h, hipStream_t st) {
-  switch (h->recon) {
-    case 1: return enqueue_general_r<T, 1>(h, st);
-    case 2: return enqueue_general_r<T, 2>(h, st);
-    case 4: return enqueue_general_r<T, 4>(h, st);
-    default: return enqueue_general_r<T, 3>(h, st);
-  }
+// k_settle (the pending mass-fixer total), the block partials into scratch (diag_scratch_words
+// doubles for ntracers) and the totals into out ([ntracers][4] device doubles); no host
+// synchronisation and no allocation, so a graph may capture it.
+int diagnostics(csadv_t *h, int solve, int tracer0, int ntracers, const double *qexact, double limit, double *out,
+                double *scratch, hipStream_t st) {
+  hipLaunchKernelGGL(k_settle, dim3(1), dim3(BLOCK), 0, st, h->dev, solve);
+  if (h->dtype == CSADV_F64)
+    hipLaunchKernelGGL(k_diag<double>, dim3(kDiagBlocks, ntracers), dim3(BLOCK), 0, st, h->dev, solve, tracer0, qexact,
+                       scratch);
+  else
+    hipLaunchKernelGGL(k_diag<float>, dim3(kDiagBlocks, ntracers), dim3(BLOCK), 0, st, h->dev, solve, tracer0, qexact,
+                       scratch);
+  hipLaunchKernelGGL(k_diag_final, dim3(ntracers), dim3(BLOCK), 0, st, h->dev, solve, scratch, limit, out);
+  HIPCHK(h, hipGetLastError());
+  return CSADV_OK;
 }
 
 // The kernels of one step, in order, on stream st (event pairs only when armed).
@@ -1577,18 +1099,32 @@ int csadv_mass(csadv_t *h, int32_t solve, int32_t tracer, double *out, void *str
   if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr || !out)
     return fail(h, CSADV_E_ARG, "mass: bad solve / tracer / output");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_settle, dim3(1), dim3(BLOCK), 0, st, h->dev, solve);
-  if (h->dtype == CSADV_F64)
-    hipLaunchKernelGGL(k_mass<double>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, h->red);
-  else
-    hipLaunchKernelGGL(k_mass<float>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, h->red);
-  HIPCHK(h, hipGetLastError());
-  std::vector<double> part(2 * kRedBlocks);
-  HIPCHK(h, hipMemcpyAsync(part.data(), h->red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+  double *dout = diag_out(h);
+  const int rc = diagnostics(h, solve, tracer, 1, nullptr, 0.0, dout, h->red, st);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpyAsync(out, dout, sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  KSum acc{0.0, 0.0};
-  for (int b = 0; b < kRedBlocks; ++b) { kadd(acc, part[2 * b]); kadd(acc, part[2 * b + 1]); }
-  out[0] = acc.s + acc.c;
+  return CSADV_OK;
+}
+
+int csadv_diagnostics(csadv_t *h, int32_t solve, int32_t tracer0, int32_t ntracers, const double *qexact,
+                      double blowup_limit, double *out, void *stream) {
+  if (!h || !h->dev) return CSADV_E_STATE;
+  const int rc = diag_args(h, solve, tracer0, ntracers, blowup_limit, out);
+  if (rc) return rc;
+  return diagnostics(h, solve, tracer0, ntracers, qexact, blowup_limit, out, h->red, (hipStream_t)stream);
+}
+
+int csadv_status(csadv_t *h, int32_t solve, int32_t *steps_done, int32_t *halted, void *stream) {
+  if (!h || !h->dev) return CSADV_E_STATE;
+  if (solve < 0 || solve >= (int)h->desc.size()) return fail(h, CSADV_E_ARG, "status: bad solve index");
+  hipStream_t st = (hipStream_t)stream;
+  int32_t ctl[2] = {0, 0};
+  HIPCHK(h, hipMemcpyAsync(ctl, h->desc[solve].ctl, sizeof(ctl), hipMemcpyDeviceToHost, st));
+  HIPCHK(h, hipStreamSynchronize(st));
+  h->done[solve] = ctl[1];
+  if (steps_done) *steps_done = ctl[1];
+  if (halted) *halted = ctl[0];
   return CSADV_OK;
 }
 
@@ -1617,26 +1153,11 @@ int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact,
   if (solve < 0 || solve >= (int)h->desc.size() || tracer < 0 || tracer >= h->desc[solve].ntr || !out || !qexact)
     return fail(h, CSADV_E_ARG, "norms: bad solve / tracer / pointer");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_settle, dim3(1), dim3(BLOCK), 0, st, h->dev, solve);
-  if (h->dtype == CSADV_F64)
-    hipLaunchKernelGGL(k_norms<double>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, qexact, h->red);
-  else
-    hipLaunchKernelGGL(k_norms<float>, dim3(kRedBlocks), dim3(BLOCK), 0, st, h->dev, solve, tracer, qexact, h->red);
-  HIPCHK(h, hipGetLastError());
-  std::vector<double> part(5 * kRedBlocks);
-  HIPCHK(h, hipMemcpyAsync(part.data(), h->red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st));
+  double *dout = diag_out(h);
+  const int rc = diagnostics(h, solve, tracer, 1, qexact, 0.0, dout, h->red, st);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpyAsync(out, dout + 1, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(h, hipStreamSynchronize(st));
-  KSum s1{0.0, 0.0}, s2{0.0, 0.0};
-  double mx = 0.0;
-  for (int b = 0; b < kRedBlocks; ++b) {
-    kadd(s1, part[5 * b]); kadd(s1, part[5 * b + 1]);
-    kadd(s2, part[5 * b + 2]); kadd(s2, part[5 * b + 3]);
-    mx = std::max(mx, part[5 * b + 4]);
-  }
-  const double n = 6.0 * h->desc[solve].N * h->desc[solve].N;
-  out[0] = mx;
-  out[1] = (s1.s + s1.c) / n;
-  out[2] = std::sqrt((s2.s + s2.c) / n);
   return CSADV_OK;
 }
 

@@ -5,27 +5,9 @@
 #include <stdint.h>
 
 #include "csadv.h"
+#include "csadv_plan.hpp"
 
 namespace csadv {
-
-constexpr int NG = 4;          // ghost layers per side (src/cs_datastruct.py:225-231)
-constexpr int BLOCK = 256;     // threads per block of the step kernel (4 waves)
-
-// Device copy of one solve: the public descriptor plus derived strides.
-struct Solve {
-  csadv_solve_desc d;
-  int32_t M;
-  int32_t strip_w, seg_l;            // one wavefront: strip_w columns x seg_l rows of a panel
-  int32_t strips, segs, items;       // strips and segments per panel, items = strips * segs
-  int64_t pstride, tstride;          // cell arrays: panel / tracer strides
-  int64_t pstride_u, pstride_v;      // x-edge / y-edge arrays: panel strides
-  int32_t part_per_tracer;           // partial sums per tracer (= 6 * items)
-  int32_t pad;
-};
-
-constexpr int WAVE = 64;
-constexpr int kXcds = 8;             // MI355X: 8 XCDs; workgroup b is dispatched to XCD b % 8
-constexpr int STRIP_MAX = WAVE - 6;  // interior columns of one wavefront (3-cell halo each side)
 
 // A set of FP64 or dtype fields that receive a duo-grid ghost fill.
 struct FieldSet {
@@ -46,6 +28,16 @@ __device__ __forceinline__ void halo_src(const int32_t *o, int d, int r, int &q,
   q = o[0];
   i = o[1] + o[2] * d + o[3] * r;
   j = o[4] + o[5] * d + o[6] * r;
+}
+
+// Side of the neighbour across (p, side) and the sign relating the two axes across the edge.
+// o = orient row {q, ib, id, ir, jb, jd, jr}; sides 0 E, 1 W, 2 N, 3 S.
+__device__ __forceinline__ void across(const int32_t *o, int side, int &sq, int &sign) {
+  const int id = o[2], jd = o[5];
+  sq = (id > 0) + (id == 0) * (2 + (jd > 0));       // id > 0: W; id < 0: E; id == 0: jd > 0 S, else N
+  const int out = (side == 0 || side == 2) ? 1 : -1;
+  const int in = (sq == 1 || sq == 3) ? 1 : -1;
+  sign = out * in;
 }
 
 // Ghost target of (side, depth d, along-edge k); sides 0 E, 1 W, 2 N, 3 S.

@@ -18,43 +18,13 @@
 // inward axis sign says whether edge values swap (q_L <-> q_R) and fluxes change sign.
 // MF-PR is applied in the step itself here (no pending correction), as the reference does.
 //
-// Included by csadv.hip (same translation unit); kernels take the solve and the tracer.
+// Included by general_f64.hip and general_f32.hip, which instantiate enqueue_general_t; kernels
+// take the solve and the tracer.
+
+#pragma once
+#include "csadv_internal.hpp"
 
 namespace gp {
-
-// ---------------------------------------------------------------- work layout
-// Element counts of one solve's work buffer (dtype T), one tracer at a time.
-struct Work {
-  int64_t cell, xe, ye, strips;          // element counts
-  int64_t o_qx, o_qy, o_pxl, o_pxr, o_pyl, o_pyr, o_fx, o_fy, o_div, o_strip, o_avg, o_red;
-  int64_t total;                         // elements of T (+ the FP64 reduction tail)
-};
-
-__host__ __device__ inline Work layout(int N, int ld, int ldu) {
-  Work w{};
-  const int M = N + 2 * NG;
-  w.cell = (int64_t)6 * M * ld;
-  w.xe = (int64_t)6 * M * ldu;
-  w.ye = (int64_t)6 * (M + 1) * ld;
-  w.strips = (int64_t)6 * 4 * 4 * M;
-  int64_t o = 0;
-  w.o_qx = o; o += w.cell;
-  w.o_qy = o; o += w.cell;
-  w.o_pxl = o; o += w.cell;
-  w.o_pxr = o; o += w.cell;
-  w.o_pyl = o; o += w.cell;
-  w.o_pyr = o; o += w.cell;
-  w.o_fx = o; o += w.xe;
-  w.o_fy = o; o += w.ye;
-  w.o_div = o; o += w.cell;
-  w.o_strip = o; o += 2 * w.strips;      // copy-halo strips (two fields)
-  w.o_avg = o; o += (int64_t)6 * 4 * M;  // cube-edge averages
-  o = (o + 7) / 8 * 8;
-  w.o_red = o;                           // FP64 partial sums, kRedGen doubles (in T units)
-  w.total = o;
-  return w;
-}
-constexpr int kRedGen = 512;             // blocks of the divergence reduction
 
 template <typename T>
 struct View {
@@ -82,11 +52,11 @@ __device__ __forceinline__ View<T> view(const Solve *solves, int s, int tr) {
   v.i0 = NG; v.iend = D.N + NG;
   v.pstride = S.pstride; v.pstride_u = S.pstride_u; v.pstride_v = S.pstride_v;
   const int steps = D.ctl[1];
-  v.cell = layout(D.N, D.ld, D.ldu).cell;
+  v.cell = work_layout(D.N, D.ld, D.ldu).cell;
   T *q0 = reinterpret_cast<T *>(D.q0) + tr * S.tstride, *q1 = reinterpret_cast<T *>(D.q1) + tr * S.tstride;
   v.qin = (steps & 1) ? q1 : q0;
   v.qout = (steps & 1) ? q0 : q1;
-  const Work w = layout(D.N, D.ld, D.ldu);
+  const WorkLayout w = work_layout(D.N, D.ld, D.ldu);
   T *b = reinterpret_cast<T *>(D.work);
   v.qx = b + w.o_qx; v.qy = b + w.o_qy;
   v.pxl = b + w.o_pxl; v.pxr = b + w.o_pxr; v.pyl = b + w.o_pyl; v.pyr = b + w.o_pyr;
@@ -96,16 +66,9 @@ __device__ __forceinline__ View<T> view(const Solve *solves, int s, int tr) {
   return v;
 }
 
-__device__ __forceinline__ bool finished(const csadv_solve_desc &D) { return D.stop > 0 && D.ctl[1] >= D.stop; }
-
-// Side of the neighbour across (p, side) and the sign relating the two axes across the edge.
-// o = orient row {q, ib, id, ir, jb, jd, jr}; sides 0 E, 1 W, 2 N, 3 S.
-__device__ __forceinline__ void across(const int32_t *o, int side, int &sq, int &sign) {
-  const int id = o[2], jd = o[5];
-  sq = (id > 0) + (id == 0) * (2 + (jd > 0));       // id > 0: W; id < 0: E; id == 0: jd > 0 S, else N
-  const int out = (side == 0 || side == 2) ? 1 : -1;
-  const int in = (sq == 1 || sq == 3) ? 1 : -1;
-  sign = out * in;
+// finished its own step count, or halted by a diagnostics call whose blow-up limit was exceeded
+__device__ __forceinline__ bool finished(const csadv_solve_desc &D) {
+  return (D.stop > 0 && D.ctl[1] >= D.stop) || D.ctl[0] != 0;
 }
 
 // cell (i, j) of panel p in a cell array
@@ -591,44 +554,100 @@ __global__ void k_gp_div_update(const Solve *solves, int s, int tr) {
   }
 }
 
-// ET-S72 / ET-PL07 wind halo for RK2 (src/edges_treatment.py:305-347), on every wind basis
-// field: the ghost edge one outside each panel side takes the neighbour's normal wind at its
-// edge one inside the shared edge, times the axis sign across the edge.
-__global__ void k_gp_wind_rk2_copy(const Solve *solves, int s) {
-  const Solve &S = solves[s];
-  const csadv_solve_desc &D = S.d;
-  const int ld = D.ld, ldu = D.ldu, i0 = NG, iend = D.N + NG;
-  const int64_t bu = 6 * S.pstride_u, bv = 6 * S.pstride_v;
-  const unsigned N = D.N, total = (unsigned)D.nbasis * 24u * N;
-  for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    const unsigned u = t / N;                        // u = (b * 6 + p) * 4 + side
-    const int k = (int)(t % N) + i0;
-    const int side = u & 3, p = (u >> 2) % 6, b = (u >> 2) / 6;
-    double *U = D.wbu + b * bu, *V = D.wbv + b * bv;
-    const int32_t *o = D.orient + (p * 4 + side) * 7;
-    int q, i, j, sq, sign;
-    halo_src(o, 0, k, q, i, j);
-    across(o, side, sq, sign);
-    // the neighbour's normal wind one edge inside the shared edge, x-edges (E/W) or y-edges (N/S);
-    // both addresses are formed and one is selected
-    const int ne = (sq == 0 || sq == 2) ? iend - 1 : i0 + 1;
-    const double *nu = U + q * S.pstride_u + (int64_t)j * ldu + ne;
-    const double *nv = V + q * S.pstride_v + (int64_t)ne * ld + i;
-    const double nb = *(sq < 2 ? nu : nv);
-    const double val = sign > 0 ? nb : -nb;
-    // our ghost edge one outside the side
-    const int te = (side == 0 || side == 2) ? iend + 1 : i0 - 1;
-    double *tu = U + p * S.pstride_u + (int64_t)k * ldu + te;
-    double *tv = V + p * S.pstride_v + (int64_t)te * ld + k;
-    *(side < 2 ? tu : tv) = val;
+}  // namespace gp
+
+
+// ET-PL07 edge values: reconstruction of the cells next to the panel sides with the
+// extrapolation, cube-edge averages, their write-back with the ghost parabolas
+template <typename T, int R>
+void gp_edges(csadv_t *h, hipStream_t st, int s, int tr, int fx_field, int fy_field) {
+  const csadv_solve_desc &d = h->desc[s];
+  const int64_t sides = (int64_t)6 * 4 * d.N;
+  hipLaunchKernelGGL((gp::k_gp_extrap<T, R>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
+                     fy_field);
+  hipLaunchKernelGGL((gp::k_gp_edge_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
+  hipLaunchKernelGGL((gp::k_gp_edge_ghosts<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
+}
+
+template <typename T>
+void gp_copy_halo(csadv_t *h, hipStream_t st, int s, int tr, int mode) {
+  const int M = h->desc[s].N + 2 * NG;
+  hipLaunchKernelGGL((gp::k_gp_copy_gather<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
+                     mode);
+  hipLaunchKernelGGL((gp::k_gp_copy_scatter<T>), dim3(gp_blocks((int64_t)96 * M)), dim3(BLOCK), 0, st, h->dev, s, tr,
+                     mode);
+}
+
+// One step of every solve by the general step (src/discrete_operators.py:18-101 stage by stage).
+template <typename T, int R>
+int enqueue_general_r(csadv_t *h, hipStream_t st) {
+  const int ns = (int)h->desc.size();
+  bool any_dg = false;
+  for (auto &d : h->desc) any_dg = any_dg || d.et == 3;
+  if (any_dg) {                  // ET-DG solves: the full duo-grid halo of Q
+    const int rc = ghost_fill(h, st);
+    if (rc) return rc;
+  }
+  for (int s = 0; s < ns; ++s) {
+    const csadv_solve_desc &d = h->desc[s];
+    const int N = d.N, M = d.N + 2 * NG;
+    const int64_t edges = (int64_t)6 * M * (N + 1), cells = (int64_t)6 * M * M;
+    for (int tr = 0; tr < d.ntr; ++tr) {
+      // ET-PL07 rewrites edge values next to the panel sides between the reconstruction and
+      // the flux, which reads those from the arrays; every other edge value the flux kernel
+      // reconstructs itself
+      const bool fuse = d.et != 2;
+      // x- and y-edge fluxes in one launch: gp_blocks(edges / 2) blocks per direction, two
+      // edge items per lane
+      auto flux = [&](int fx_field, int fy_field) {
+        const int nbx = gp_blocks((edges + 1) / 2);
+        if (fuse)
+          hipLaunchKernelGGL((gp::k_gp_flux<T, R, true>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
+                             fy_field, nbx);
+        else
+          hipLaunchKernelGGL((gp::k_gp_flux<T, R, false>), dim3(2 * nbx), dim3(BLOCK), 0, st, h->dev, s, tr, fx_field,
+                             fy_field, nbx);
+      };
+      if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 0);
+      if (!fuse) {
+        gp_edges<T, R>(h, st, s, tr, 0, 0);
+      }
+      flux(0, 0);
+      hipLaunchKernelGGL((gp::k_gp_split<T>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      if (d.et != 3) gp_copy_halo<T>(h, st, s, tr, 1);
+      // second pass with the arguments swapped: x-fluxes of Qy, y-fluxes of Qx
+      if (!fuse) {
+        gp_edges<T, R>(h, st, s, tr, 2, 1);
+      }
+      flux(2, 1);
+      if (d.mf == 2) {           // MF-AF: gather the cube-edge averages; k_gp_div_update reads them
+        const int64_t sides = (int64_t)6 * 4 * N;
+        hipLaunchKernelGGL((gp::k_gp_flux_avg<T>), dim3(gp_blocks(sides)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      }
+      if (d.mf == 3) {           // MF-PR: the global sum of div g sits between div and the update
+        hipLaunchKernelGGL((gp::k_gp_div<T>), dim3(kRedGen), dim3(BLOCK), 0, st, h->dev, s, tr);
+        hipLaunchKernelGGL((gp::k_gp_update<T>), dim3(gp_blocks(cells)), dim3(BLOCK), 0, st, h->dev, s, tr);
+      } else {
+        if (d.mf == 2)
+          hipLaunchKernelGGL((gp::k_gp_div_update<T, true>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev,
+                             s, tr);
+        else
+          hipLaunchKernelGGL((gp::k_gp_div_update<T, false>), dim3(gp_blocks((cells + 1) / 2)), dim3(BLOCK), 0, st, h->dev,
+                             s, tr);
+      }
+    }
+    launch_gp_advance(h, st, s);
+  }
+  HIPCHK(h, hipGetLastError());
+  return CSADV_OK;
+}
+
+template <typename T>
+int enqueue_general_t(csadv_t *h, hipStream_t st) {
+  switch (h->recon) {
+    case 1: return enqueue_general_r<T, 1>(h, st);
+    case 2: return enqueue_general_r<T, 2>(h, st);
+    case 4: return enqueue_general_r<T, 4>(h, st);
+    default: return enqueue_general_r<T, 3>(h, st);
   }
 }
-
-// the step is complete for every tracer: advance the step counter
-__global__ void k_gp_advance(const Solve *solves, int s) {
-  const csadv_solve_desc &D = solves[s].d;
-  if (D.stop > 0 && D.ctl[1] >= D.stop) return;
-  if (threadIdx.x == 0) D.ctl[1] = D.ctl[1] + 1;
-}
-
-}  // namespace gp

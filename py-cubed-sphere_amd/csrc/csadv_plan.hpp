@@ -1,0 +1,115 @@
+// Host-side planning of libcsadv: descriptor validation, tile shapes, the XCD-banded work list
+// of the step kernel and the general step's work-buffer layout.  Plain C++17 with no HIP
+// dependency, so that it builds with g++ under the address and undefined-behaviour sanitizers
+// on its own; the kernels share its layout structs and constants.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "csadv.h"
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#define CSADV_HD __host__ __device__
+#else
+#define CSADV_HD
+#endif
+
+namespace csadv {
+
+constexpr int NG = 4;                // ghost layers per side (src/cs_datastruct.py:225-231)
+constexpr int BLOCK = 256;           // threads per block of the whole-array kernels (4 waves)
+constexpr int WAVE = 64;             // CDNA4 wavefront
+constexpr int kXcds = 8;             // MI355X: 8 XCDs; workgroup b is dispatched to XCD b % 8
+constexpr int STRIP_MAX = WAVE - 6;  // interior columns of one wavefront (3-cell halo each side)
+constexpr int kRedGen = 512;         // blocks of the general step's divergence reduction
+constexpr int kDiagBlocks = 256;     // blocks of a diagnostics reduction (partials per tracer)
+constexpr int kDiagWords = 8;        // doubles per diagnostics partial
+constexpr int kMaxTracers = 64;      // tracers per solve
+
+// tile id -> rows per wavefront segment (the strip width is always the widest that fits)
+constexpr int kSegRows[] = {8, 4, 16, 32, 6, 12, 11, 5, 10, 14, 17, 23};
+constexpr int kNumTiles = sizeof(kSegRows) / sizeof(kSegRows[0]);
+
+// Device copy of one solve: the public descriptor plus derived strides.
+struct Solve {
+  csadv_solve_desc d;
+  int32_t M;
+  int32_t strip_w, seg_l;            // one wavefront: strip_w columns x seg_l rows of a panel
+  int32_t strips, segs, items;       // strips and segments per panel, items = strips * segs
+  int64_t pstride, tstride;          // cell arrays: panel / tracer strides
+  int64_t pstride_u, pstride_v;      // x-edge / y-edge arrays: panel strides
+  int32_t part_per_tracer;           // partial sums per tracer (= 6 * items)
+  int32_t pad;
+};
+
+// One work item of the step kernel: (solve, panel + 8 * tracer, strip, segment); solve -1 pads.
+// Same layout as HIP's int4.
+struct Item {
+  int32_t x, y, z, w;
+};
+
+// General step: element counts of one solve's work buffer (dtype T), one tracer at a time.
+struct WorkLayout {
+  int64_t cell, xe, ye, strips;          // element counts
+  int64_t o_qx, o_qy, o_pxl, o_pxr, o_pyl, o_pyr, o_fx, o_fy, o_div, o_strip, o_avg, o_red;
+  int64_t total;                         // elements of T (+ the FP64 reduction tail)
+};
+
+CSADV_HD inline WorkLayout work_layout(int N, int ld, int ldu) {
+  WorkLayout w{};
+  const int M = N + 2 * NG;
+  w.cell = (int64_t)6 * M * ld;
+  w.xe = (int64_t)6 * M * ldu;
+  w.ye = (int64_t)6 * (M + 1) * ld;
+  w.strips = (int64_t)6 * 4 * 4 * M;
+  int64_t o = 0;
+  w.o_qx = o; o += w.cell;
+  w.o_qy = o; o += w.cell;
+  w.o_pxl = o; o += w.cell;
+  w.o_pxr = o; o += w.cell;
+  w.o_pyl = o; o += w.cell;
+  w.o_pyr = o; o += w.cell;
+  w.o_fx = o; o += w.xe;
+  w.o_fy = o; o += w.ye;
+  w.o_div = o; o += w.cell;
+  w.o_strip = o; o += 2 * w.strips;      // copy-halo strips (two fields)
+  w.o_avg = o; o += (int64_t)6 * 4 * M;  // cube-edge averages
+  o = (o + 7) / 8 * 8;
+  w.o_red = o;                           // FP64 partial sums, kRedGen doubles (in T units)
+  w.total = o;
+  return w;
+}
+
+// Row strides the general step expects (and csadv_work_bytes sizes its buffer for).
+inline int general_ld(int N) { return (N + 2 * NG + 3) / 4 * 4; }
+inline int general_ldu(int N) { return (N + 2 * NG + 4) / 4 * 4; }
+
+// The option pairs of one solve that need the general step (ET-S72, ET-PL07, MF-AF).
+inline bool is_general(const csadv_solve_desc &d) { return d.et != 3 || d.mf == 2; }
+
+// Descriptor checks of csadv_create / csadv_update: CSADV_OK, or an error code and message.
+int validate(const csadv_solve_desc &d, std::string &msg);
+
+// Solves of one handle share dtype, recon, split, dp and the step kind.
+int check_compatible(const csadv_solve_desc *solves, int nsolve, std::string &msg);
+
+// Strips (columns) and segments (rows) per panel of tile `tile` at size N.
+void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs);
+
+// The descriptor with its derived strides; tile = the handle's tile when d.tile < 0.
+Solve derive(const csadv_solve_desc &d, int tile);
+
+// The step kernel's work list: every (solve, panel, tracer, strip, segment) once, the row
+// segments of each solve cut into kXcds contiguous bands, one per XCD, and the bands
+// interleaved so that workgroup b (on XCD b % kXcds) takes its item from band b % kXcds
+// (padded with solve -1 to equal length).
+std::vector<Item> work_list(const std::vector<Solve> &solves);
+
+// Sizes of the public entry points (see include/csadv.h).
+int partials_per_tracer(int N, int tile);
+int work_bytes(int N, int dtype, int64_t *bytes);
+int tile_rows(int tile);
+
+}  // namespace csadv

@@ -1,0 +1,4 @@
+// The general step (ET-S72, ET-PL07, MF-AF) of FP64: one translation unit per precision.
+#include "csadv_general.hpp"
+
+template int enqueue_general_t<double>(csadv_t *h, hipStream_t st);

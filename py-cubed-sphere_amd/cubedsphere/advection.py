@@ -24,6 +24,7 @@ remaps and the reference's data files; it draws no plots (cartopy is not availab
 are those of src/errors.py:99-113 at the final step.
 """
 import numpy as np
+import torch
 
 from . import config, seam
 from ._lib import CsadvError
@@ -98,16 +99,37 @@ def update_adv(cs_grid, simulation, t):
     simulation._wind_t = float(t)
 
 
+BLOWUP_LINF = 100.0     # src/output.py:44-47: a run stops once its L_inf error exceeds this
+CHECK_EVERY = 16        # steps between the device's blow-up checks in adv_sphere
+
+
 def adv_sphere(cs_grid, simulation, divtest_flag=False, datadir=None, verbose=False):
     """src/advection_sphere.py:13-61 without plotting: run Tf/dt steps, return the final
     (L_inf, L_1, L_2) errors against the exact solution.  Nothing is read between steps
-    (there is no per-step output here), so the steps go to the device in one call, which is
-    the same as Nsteps calls of adv_time_step + update_adv."""
+    (there is no per-step output here), so the steps go to the device in chunks of 16 (one
+    replayed graph each) with no host synchronisation, which is the same as Nsteps calls of
+    adv_time_step + update_adv.  After every chunk the device checks the field itself
+    (csadv_diagnostics): a max |Q| above 100 + max |Q(0)| means an L_inf error above 100, and
+    halts the solve there; the run then stops as the reference's does (src/output.py:44-47),
+    raising CsadvError, and so does a final L_inf error above 100."""
     nsteps = 1 if divtest_flag else int(simulation.Tf / simulation.dt)
     init_vars_adv(cs_grid, simulation)
     if nsteps >= 64:
-        simulation.use_graph(16)          # replay 16-step graphs: no per-launch host cost
-    simulation.step(nsteps)
+        simulation.use_graph(CHECK_EVERY)    # replay graphs: no per-launch host cost
+    limit = BLOWUP_LINF + float(simulation.diagnostics()[:, 1].max())
+    check = torch.empty((simulation.case.ntr, 4), dtype=torch.float64, device=simulation.device)
+    done = 0
+    while done < nsteps:
+        k = min(CHECK_EVERY, nsteps - done)
+        simulation.step(k)
+        simulation.diagnostics(blowup_limit=limit, out=check)
+        done += k
+    steps, halted = simulation.status()
+    linf = simulation.norms(t=steps * simulation.dt)[0]
+    if halted or linf > BLOWUP_LINF:
+        print("Stopping due to large errors.")
+        print("The CFL number is:", getattr(simulation, "CFL", None))
+        raise CsadvError("advection blew up (L_inf error above %g) by step %d of %d" % (BLOWUP_LINF, steps, nsteps))
     if datadir is not None or verbose:       # src/output.py at the final step
         return output_adv(cs_grid, simulation, nsteps, nsteps, datadir=datadir, verbose=verbose)[0]
     simulation.mass_change = _mass_change(simulation.total_mass0, simulation.mass())

@@ -437,6 +437,32 @@ class Advection:
                    "csadv_mass")
         return out.value
 
+    def diagnostics(self, qexact=None, blowup_limit=0.0, out=None, tracer0=0, ntracers=None):
+        """Device diagnostics of the tracers, queued on the stream without a host synchronisation
+        (graph-capturable): a float64 device tensor [ntracers][4] of (mass, Linf, L1, L2) per
+        tracer (src/diagnostics.py:14-26, src/errors.py:99-113), the norms against ``qexact``
+        (device FP64 [ntracers][6][N][N]) or of the field itself.  ``blowup_limit`` > 0 halts the
+        solve when a Linf exceeds it (src/output.py:44-47); ``status()`` reports it."""
+        n = self.case.ntr - tracer0 if ntracers is None else int(ntracers)
+        if out is None:
+            out = torch.empty((n, 4), dtype=torch.float64, device=self.device)
+        qe = None
+        if qexact is not None:
+            qe = qexact if torch.is_tensor(qexact) else torch.from_numpy(np.ascontiguousarray(qexact))
+            qe = qe.to(self.device, torch.float64).contiguous()
+            self._diag_exact = qe            # kept alive until the queued kernels have read it
+        _lib.check(self.lib.csadv_diagnostics(self.handle, self.index, int(tracer0), n, _ptr(qe), float(blowup_limit),
+                                              _ptr(out), self._stream()), self.handle, "csadv_diagnostics")
+        return out
+
+    def status(self):
+        """(steps completed, halted) from the device; the host's step count follows it."""
+        steps, halted = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(self.lib.csadv_status(self.handle, self.index, ctypes.byref(steps), ctypes.byref(halted),
+                                         self._stream()), self.handle, "csadv_status")
+        self.steps_done = steps.value
+        return steps.value, bool(halted.value)
+
     def exact(self, tracer=0, t=None):
         g = self.grid
         t = self.steps_done * self.dt if t is None else t
