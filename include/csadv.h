@@ -176,6 +176,16 @@ int csadv_norms(csadv_t *h, int32_t solve, int32_t tracer, const double *qexact,
 int csadv_diagnostics(csadv_t *h, int32_t solve, int32_t tracer0, int32_t ntracers, const double *qexact,
                       double blowup_limit, double *out, void *stream);
 
+/* csadv_diagnostics with its block partials in a fixed device array of the library instead of the
+ * handle's scratch, so that the call allocates nothing and needs no scratch of its own: at most 16
+ * tracers per call (CSADV_E_RANGE past that; the array holds 256 KiB on each GPU until the process
+ * ends).  Same kernels, same results as csadv_diagnostics.  Calls that overlap on one GPU (on two
+ * streams at once) share the array and race.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). */
+int csadv_diagnostics_noalloc(csadv_t *h, int32_t solve, int32_t tracer0, int32_t ntracers, const double *qexact,
+                              double blowup_limit, double *out, void *stream);
+
 /* Steps completed and the halt flag of one solve, read from the device (synchronises the stream);
  * the host's count of completed steps (csadv_current) follows the device after a halt.  No
  * counterpart in the reference (its loop exits on a blow-up). */

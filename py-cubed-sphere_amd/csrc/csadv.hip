@@ -966,6 +966,31 @@ int csadv_ghost_fill(csadv_t *h, void *stream) {
 
 namespace {
 
+// Scratch of csadv_diagnostics_noalloc: the block partials of up to kScratchTracers tracers in a
+// file-scope device array (256 KiB on every GPU the library runs on, held until the process
+// ends).  In host code `scratch` is only a stand-in for the symbol: its device address comes
+// from hipGetSymbolAddress, once per device, and only that address is handed to kernels.  Every
+// kernel of a call writes all the partials it reads, so nothing has to start at zero; calls that
+// overlap on one GPU (two streams at once) share the array and race.
+constexpr int kScratchTracers = 16;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[(int64_t)kScratchTracers * kDiagBlocks * kDiagWords];
+double *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up on its first use there).
+int scratch_address(csadv_t *h, double **ptr) {
+  int dev = -1;
+  HIPCHK(h, hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(h, CSADV_E_RANGE, "diagnostics scratch: device index out of range");
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    HIPCHK(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch[dev];
+  return CSADV_OK;
+}
+
 // Device outputs of csadv_mass / csadv_norms: after the partials of the handle's largest solve.
 double *diag_out(csadv_t *h) { return h->red + (int64_t)h->red_tracers * kDiagBlocks * kDiagWords; }
 
@@ -1113,6 +1138,20 @@ int csadv_diagnostics(csadv_t *h, int32_t solve, int32_t tracer0, int32_t ntrace
   const int rc = diag_args(h, solve, tracer0, ntracers, blowup_limit, out);
   if (rc) return rc;
   return diagnostics(h, solve, tracer0, ntracers, qexact, blowup_limit, out, h->red, (hipStream_t)stream);
+}
+
+int csadv_diagnostics_noalloc(csadv_t *h, int32_t solve, int32_t tracer0, int32_t ntracers, const double *qexact,
+                              double blowup_limit, double *out, void *stream) {
+  if (!h || !h->dev) return CSADV_E_STATE;
+  int rc = diag_args(h, solve, tracer0, ntracers, blowup_limit, out);
+  if (rc) return rc;
+  if (ntracers > kScratchTracers)
+    return fail(h, CSADV_E_RANGE, "csadv_diagnostics_noalloc: at most " + std::to_string(kScratchTracers) +
+                                      " tracers per call (its scratch is a fixed device array)");
+  double *ptr = nullptr;
+  rc = scratch_address(h, &ptr);
+  if (rc) return rc;
+  return diagnostics(h, solve, tracer0, ntracers, qexact, blowup_limit, out, ptr, (hipStream_t)stream);
 }
 
 int csadv_status(csadv_t *h, int32_t solve, int32_t *steps_done, int32_t *halted, void *stream) {

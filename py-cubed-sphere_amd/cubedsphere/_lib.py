@@ -17,7 +17,7 @@ _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of ord
 SYMBOLS = (
     "csadv_create", "csadv_partials_per_tracer", "csadv_work_bytes", "csadv_tile_rows", "csadv_update", "csadv_wind_init",
     "csadv_ghost_fill", "csadv_step", "csadv_set_graph", "csadv_finalize", "csadv_mass", "csadv_norms",
-    "csadv_diagnostics", "csadv_status",
+    "csadv_diagnostics", "csadv_diagnostics_noalloc", "csadv_status",
     "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_time_step_kernel", "csadv_remap_latlon", "csadv_last_error", "csadv_destroy",
     "csadv_abi_version",
 )
@@ -76,6 +76,7 @@ def load():
     lib.csadv_mass.argtypes = [vp, i32, i32, dbl, vp]
     lib.csadv_norms.argtypes = [vp, i32, i32, vp, dbl, vp]
     lib.csadv_diagnostics.argtypes = [vp, i32, i32, i32, vp, ctypes.c_double, vp, vp]
+    lib.csadv_diagnostics_noalloc.argtypes = [vp, i32, i32, i32, vp, ctypes.c_double, vp, vp]
     lib.csadv_status.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), vp]
     lib.csadv_current.argtypes = [vp, i32]
     lib.csadv_profile.argtypes = [vp, i32]
