@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CSADV_ABI_VERSION 6
+#define CSADV_ABI_VERSION 7
 
 enum csadv_dtype { CSADV_F64 = 0, CSADV_F32 = 1 };
 
@@ -100,6 +100,11 @@ typedef struct csadv_solve_desc {
                                       number of steps while its solves end at their own step);
                                       0 = no limit                                           */
   void *work;                      /* general step only: scratch of csadv_work_bytes() bytes */
+  double *offset;                  /* fused step, FP32: [ntr] constants (exact in FP32) the stored
+                                      state is kept from, Q = S + C gfix + offset, so that
+                                      increments far below an ulp of Q (on a flat background)
+                                      are not rounded away; NULL = 0.  csadv_finalize folds them
+                                      into the state and zeroes them.  NULL for the general step */
 } csadv_solve_desc;
 
 typedef struct csadv_s csadv_t;

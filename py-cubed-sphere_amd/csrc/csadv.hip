@@ -292,25 +292,30 @@ __global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
   ghost_cells<T>(one, red);
 }
 
-// Fold the correction totals into the current buffer (interior only); k_clear_corr then
-// clears them.
+// Fold the correction totals (interior) and the FP32 state offsets (every cell: a ghost value is
+// a weighted sum with weights summing to 1, so it takes the offset as it is) into the current
+// buffer; k_clear_corr then clears them.
 template <typename T>
 __global__ void k_finalize(Solve *solves, int first) {
   Solve &S = solves[first + blockIdx.y];
   csadv_solve_desc &D = S.d;
-  if (D.mf != 3) return;
-  const int N = D.N, ld = D.ld;
+  if (D.mf != 3 && !D.offset) return;
+  const int N = D.N, M = S.M, ld = D.ld;
   T *Q = reinterpret_cast<T *>((D.ctl[1] & 1) ? D.q1 : D.q0);
-  const int64_t tot = (int64_t)6 * N * N;
+  const int64_t tot = (int64_t)6 * M * M, mm = (int64_t)M * M;
   for (int tr = 0; tr < D.ntr; ++tr) {
-    const double cf = D.corr[(D.ctl[1] & 1) * D.ntr + tr];
-    if (cf == 0.0) continue;
+    const double cf = D.mf == 3 ? D.corr[(D.ctl[1] & 1) * D.ntr + tr] : 0.0;
+    const double off = D.offset ? D.offset[tr] : 0.0;
+    if (cf == 0.0 && off == 0.0) continue;
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * blockDim.x) {
-      const int p = (int)(idx / ((int64_t)N * N));
-      const int rem = (int)(idx - (int64_t)p * N * N);
-      const int j = rem / N + NG, i = rem % N + NG;
+      const int p = (int)(idx / mm);
+      const int rem = (int)(idx - p * mm);
+      const int j = rem / M, i = rem % M;
+      const bool interior = i >= NG && i < N + NG && j >= NG && j < N + NG;
+      if (!interior && off == 0.0) continue;
       T &v = Q[tr * S.tstride + p * S.pstride + (int64_t)j * ld + i];
-      v = (T)((double)v + cf * D.gc64[(int64_t)j * ld + i]);
+      const double c = interior ? cf * D.gc64[(int64_t)j * ld + i] : 0.0;
+      v = (T)(((double)v + off) + c);
     }
   }
 }
@@ -319,6 +324,8 @@ __global__ void k_clear_corr(Solve *solves, int first) {
   csadv_solve_desc &D = solves[first + blockIdx.x].d;
   if (D.mf == 3)
     for (int k = threadIdx.x; k < 2 * D.ntr; k += blockDim.x) D.corr[k] = 0.0;
+  if (D.offset)
+    for (int k = threadIdx.x; k < D.ntr; k += blockDim.x) D.offset[k] = 0.0;
 }
 
 // ------------------------------------------------------------------ lat-lon output remap
@@ -341,6 +348,7 @@ __global__ void k_remap_latlon(const Solve *solves, int solve, int tracer, const
   const double lon_step = (pi - (-pi)) / (nlon - 1), lat_step = (pi * 0.5 - (-pi * 0.5)) / (nlat - 1);
   const T *Q = reinterpret_cast<const T *>((D.ctl[1] & 1) ? D.q1 : D.q0) + tracer * S.tstride;
   const double cf = (D.corr && D.mf == 3) ? D.corr[(D.ctl[1] & 1) * D.ntr + tracer] : 0.0;
+  const double off = D.offset ? D.offset[tracer] : 0.0;
   const int64_t total = (int64_t)nlon * nlat;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
     const int ia = (int)(k / nlat), ib = (int)(k - (int64_t)ia * nlat);
@@ -369,7 +377,7 @@ __global__ void k_remap_latlon(const Solve *solves, int solve, int tracer, const
         v = field[((int64_t)p * N + j) * N + i];
       } else {
         const int ci = i + NG, cj = j + NG;
-        v = (double)Q[p * S.pstride + (int64_t)cj * ld + ci] + cf * D.gc64[(int64_t)cj * ld + ci];
+        v = ((double)Q[p * S.pstride + (int64_t)cj * ld + ci] + off) + cf * D.gc64[(int64_t)cj * ld + ci];
       }
       out[k] = v;
     }
@@ -610,7 +618,7 @@ __device__ __forceinline__ void diag_block(KSum m, KSum a1, KSum a2, double mx, 
 // tracers): per block the compensated FP64 sums of the mass Q g dx dy (src/diagnostics.py:14-26)
 // and of |e| and e^2, and the max |e| (src/errors.py:99-113), with e = qexact - Q when the exact
 // field is given ([tracers][6][N][N] FP64) and e = Q otherwise.  Q = S + C g (the pending
-// mass-fixer total), read from the solve's current buffer.
+// mass-fixer total; + the FP32 state offset), read from the solve's current buffer.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_diag(const Solve *solves, int solve, int tracer0, const double *qex,
                                                 double *part) {
@@ -621,6 +629,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag(const Solve *solves, int solve, 
   const int cur = D.ctl[1] & 1;
   const T *Q = reinterpret_cast<const T *>(cur ? D.q1 : D.q0) + tr * S.tstride;
   const double cf = (D.corr && D.mf == 3) ? D.corr[cur * D.ntr + tr] : 0.0;
+  const double off = D.offset ? D.offset[tr] : 0.0;
   const double dxdy = D.dx * D.dy;
   const int64_t cells = (int64_t)6 * N * N, nn = (int64_t)N * N;
   const double *qe = qex ? qex + (int64_t)blockIdx.y * cells : nullptr;
@@ -631,7 +640,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag(const Solve *solves, int solve, 
     const int rem = (int)(idx - p * nn);
     const int j = rem / N + NG, i = rem % N + NG;
     const double g = D.gc64[(int64_t)j * ld + i];
-    const double q = (double)Q[p * S.pstride + (int64_t)j * ld + i] + cf * g;
+    const double q = ((double)Q[p * S.pstride + (int64_t)j * ld + i] + off) + cf * g;
     kadd(m, q * g * dxdy);
     const double e = fabs(qe ? qe[idx] - q : q);
     kadd(a1, e);

@@ -38,6 +38,7 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
   if (d.dp == 2 && (!d.usgn || !d.vsgn)) { msg = "dp=2 needs the usgn and vsgn wind arrays"; return CSADV_E_ARG; }
   if (d.mf == 3 && (!d.corr || !d.partials)) { msg = "MF-PR needs corr and partials"; return CSADV_E_ARG; }
   if (!is_general(d) && !d.gfix) { msg = "the fused step (ET-DG) needs gfix"; return CSADV_E_ARG; }
+  if (is_general(d) && d.offset) { msg = "state offsets apply to the fused step only (desc.offset must be NULL)"; return CSADV_E_ARG; }
   if (d.n_strip != 6 * 4 * 3 * d.N || d.n_corner != 6 * 2 * 3 * 6 || !d.strip_tgt || !d.strip_src || !d.strip_w ||
       !d.corner_tgt || !d.corner_src || !d.corner_w) {
     msg = "ghost recipes missing or of the wrong size (expected 72 N strip and 216 corner cells)";

@@ -52,6 +52,7 @@ struct March {
   bool out_lane, need_col;
   T dtdx, dtdy;
   double cf;
+  T off;                     // FP32: the tracer's state offset (Q = S + C gfix + off); FP64: 0
   // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6).
   // The step-invariant row inputs (metric, 1/g, winds) are loaded straight into the slot their
   // row will hold: with PF = 2 the loads of row y+2 are issued at the end of iteration y, when
@@ -114,7 +115,8 @@ struct March {
   // (with no correction total, cf = 0 and q + 0 gfix = q: no branch on the mass fixer)
   __device__ __forceinline__ void correct(StepIn<T> &r) const {
     r.raw = r.q;
-    r.q = r.q + T(cf) * r.gf;
+    if constexpr (std::is_same<T, float>::value) r.q = (r.q + off) + T(cf) * r.gf;
+    else r.q = r.q + T(cf) * r.gf;
   }
 
   // Iteration y in phase K.  STAGE: how much of the column stage has begun, fixed by the row's
@@ -295,6 +297,7 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
     step = D.ctl[2];
   }
   const double c0 = mfpr ? D.corr[tr] : 0.0, c1 = mfpr ? D.corr[D.ntr + tr] : 0.0;
+  m.off = (std::is_same<T, float>::value && D.offset) ? T(D.offset[tr]) : T(0);
   if (step < 0 || (D.stop > 0 && step >= D.stop)) return;   // solve halted or finished
   const bool odd = step & 1;                  // the state of step n lives in q[n & 1]
   m.resolve(b, odd);
@@ -320,11 +323,11 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
 // descriptor in global memory the compiler reads the words with vector loads, which warm no
 // scalar cache: the multi-solve kernel does not use it.)
 __device__ __forceinline__ int touch_solve(const Solve &S, int v) {
-  static_assert(sizeof(Solve) == 424, "one word in each 64-byte line of the descriptor");
+  static_assert(sizeof(Solve) == 432, "one word in each 64-byte line of the descriptor");
   const char *b = reinterpret_cast<const char *>(&S);
   int w[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) __builtin_memcpy(&w[k], b + (k < 7 ? 64 * k : 420), 4);
+  for (int k = 0; k < 8; ++k) __builtin_memcpy(&w[k], b + (k < 7 ? 64 * k : 428), 4);
   asm("" : "=s"(v) : "0"(v), "s"(w[0]), "s"(w[1]), "s"(w[2]), "s"(w[3]), "s"(w[4]), "s"(w[5]), "s"(w[6]), "s"(w[7]));
   return v;
 }

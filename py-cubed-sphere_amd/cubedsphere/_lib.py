@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcsadv.so")
 
 CSADV_F64, CSADV_F32 = 0, 1
-ABI_VERSION = 6          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
+ABI_VERSION = 7          # CSADV_ABI_VERSION of include/csadv.h this binding mirrors
 _ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "call out of order", -4: "size out of range"}
 
 # exported symbols, in declaration order of include/csadv.h
@@ -43,7 +43,7 @@ class SolveDesc(ctypes.Structure):
         ("strip_tgt", _p), ("strip_src", _p), ("strip_w", _p),
         ("corner_tgt", _p), ("corner_src", _p), ("corner_w", _p),
         ("coef", _p), ("corr", _p), ("partials", _p), ("ctl", _p),
-        ("tile", ctypes.c_int32), ("stop", ctypes.c_int32), ("work", _p),
+        ("tile", ctypes.c_int32), ("stop", ctypes.c_int32), ("work", _p), ("offset", _p),
     ]
 
 

@@ -210,7 +210,7 @@ def test_library_loads_and_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert set(declared) == set(_lib.SYMBOLS)
     loaded = _lib.load()
-    assert loaded.csadv_abi_version() == _lib.ABI_VERSION == 6
+    assert loaded.csadv_abi_version() == _lib.ABI_VERSION == 7
     assert loaded.csadv_partials_per_tracer(384, 0) > 0
     assert loaded.csadv_partials_per_tracer(384, 99) < 0
     rows = []
