@@ -79,9 +79,8 @@ def options_string(opts):
 
 
 def is_general(opts):
-    """The general step runs every option pair the fused step kernel does not: ET-S72, ET-PL07
-    and MF-AF."""
-    return opts.get("et", 3) != 3 or opts.get("mf", 3) == 2
+    """The general step runs the options the fused step kernel does not: ET-S72 and ET-PL07."""
+    return opts.get("et", 3) != 3
 SERVED_REQUESTS, SERVED_RATE = 200, 12.0      # Poisson stream: count, arrivals per second
 BATCH_NS = (32, 48, 96, 192, 384, 512)
 # tile ids from the autotuner for the single-solve workloads (the batched workload tunes

@@ -47,9 +47,9 @@ enum csadv_status {
  *   et    1 ET-S72 | 2 ET-PL07 | 3 ET-DG
  *   mt    1 MT-0  | 2 MT-PL07   (valid pairs: split 1|2 with mt 1, split 3 with mt 2)
  *   mf    1 none  | 2 MF-AF     | 3 MF-PR
- * ET-DG without MF-AF runs the fused step kernel; ET-S72, ET-PL07 and MF-AF run the general
- * step (a kernel per stage), which needs the work buffer.  All solves of one handle take
- * the same one of the two.                                                             */
+ * ET-DG runs the fused step kernel (with MF-AF: plus a fix-up of the cube-edge cells);
+ * ET-S72 and ET-PL07 run the general step (a kernel per stage).  Both need the work buffer
+ * where noted below.  All solves of one handle take the same one of the two.            */
 typedef struct csadv_solve_desc {
   int32_t N, ntr, dtype;
   int32_t ld, ldu;                 /* row strides (elements) of cell / x-edge arrays   */
@@ -99,7 +99,9 @@ typedef struct csadv_solve_desc {
                                       has completed `stop` steps (a batch can then run a fixed
                                       number of steps while its solves end at their own step);
                                       0 = no limit                                           */
-  void *work;                      /* general step only: scratch of csadv_work_bytes() bytes */
+  void *work;                      /* general step (ET-S72, ET-PL07): scratch of csadv_work_bytes()
+                                      bytes; fused step with MF-AF: ntr * 24 * N elements of the
+                                      dtype (the second pass's cube-edge fluxes); else unused */
   double *offset;                  /* fused step, FP32: [ntr] constants (exact in FP32) the stored
                                       state is kept from, Q = S + C gfix + offset, so that
                                       increments far below an ulp of Q (on a flat background)
@@ -118,7 +120,7 @@ int csadv_create(const csadv_solve_desc *solves, int32_t nsolve, int32_t tile, c
 /* Number of FP64 partial sums the step writes per tracer (size of desc.partials / ntr). */
 int csadv_partials_per_tracer(int32_t N, int32_t tile);
 
-/* Bytes of the work buffer the general step (ET-S72, ET-PL07, MF-AF) needs for one solve of
+/* Bytes of the work buffer the general step (ET-S72, ET-PL07) needs for one solve of
  * size N and dtype (written to *bytes).  No counterpart in the reference (its scratch arrays
  * are members of the simulation object, src/cs_datastruct.py:635-684). */
 int csadv_work_bytes(int32_t N, int32_t dtype, int64_t *bytes);

@@ -26,7 +26,8 @@ struct csadv_s {
   int recon = 3;
   bool rk2 = false;               // dp = 2 (separate upwind-side winds)
   int split = 1;                  // splitting of every solve (its metric treatment follows)
-  bool general = false;           // ET-S72 / ET-PL07 / MF-AF: the general step
+  bool general = false;           // ET-S72 / ET-PL07: the general step
+  bool mfaf = false;              // a fused-step solve with MF-AF: k_af_fix after the step kernel
   bool wind1 = false;             // every solve RK1 with a one-field time-dependent wind: the
                                   // step kernel forms its winds (no per-step k_wind_avg)
   double *red = nullptr;          // diagnostics scratch: partials and outputs (diag_scratch_words)

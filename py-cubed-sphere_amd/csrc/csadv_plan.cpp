@@ -21,7 +21,11 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
   if (d.et < 1 || d.et > 3) { msg = "et must be 1 (ET-S72), 2 (ET-PL07) or 3 (ET-DG)"; return CSADV_E_ARG; }
   if (d.mf < 1 || d.mf > 3) { msg = "mf must be 1 (none), 2 (MF-AF) or 3 (MF-PR)"; return CSADV_E_ARG; }
   if (is_general(d) && !d.work) {
-    msg = "ET-S72, ET-PL07 and MF-AF run the general step, which needs desc.work (csadv_work_bytes)";
+    msg = "ET-S72 and ET-PL07 run the general step, which needs desc.work (csadv_work_bytes)";
+    return CSADV_E_ARG;
+  }
+  if (!is_general(d) && d.mf == 2 && !d.work) {
+    msg = "ET-DG with MF-AF needs desc.work for the cube-edge fluxes (ntr * 24 * N elements of the dtype)";
     return CSADV_E_ARG;
   }
   if (is_general(d) && (d.ld != general_ld(d.N) || d.ldu != general_ldu(d.N))) {

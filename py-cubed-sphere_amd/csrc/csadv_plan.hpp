@@ -86,8 +86,15 @@ CSADV_HD inline WorkLayout work_layout(int N, int ld, int ldu) {
 inline int general_ld(int N) { return (N + 2 * NG + 3) / 4 * 4; }
 inline int general_ldu(int N) { return (N + 2 * NG + 4) / 4 * 4; }
 
-// The option pairs of one solve that need the general step (ET-S72, ET-PL07, MF-AF).
-inline bool is_general(const csadv_solve_desc &d) { return d.et != 3 || d.mf == 2; }
+// The options of one solve that need the general step: ET-S72 and ET-PL07 (ET-DG runs the
+// fused step, with MF-AF through its cube-edge fix-up).
+inline bool is_general(const csadv_solve_desc &d) { return d.et != 3; }
+
+// Elements (of the state's dtype) of the work buffer the fused step needs: MF-AF keeps the
+// second pass's cube-edge fluxes, [ntr][6][4][N]; none otherwise.
+inline int64_t fused_work_elems(const csadv_solve_desc &d) {
+  return d.mf == 2 ? (int64_t)d.ntr * 6 * 4 * d.N : 0;
+}
 
 // Descriptor checks of csadv_create / csadv_update: CSADV_OK, or an error code and message.
 int validate(const csadv_solve_desc &d, std::string &msg);

@@ -53,6 +53,12 @@ struct March {
   T dtdx, dtdy;
   double cf;
   T off;                     // FP32: the tracer's state offset (Q = S + C gfix + off); FP64: 0
+  // MF-AF: the second pass's fluxes through the cube edges go to fside ([4][N]: E, W, N, S of
+  // this panel and tracer), for k_af_fix to replace them by the cube-edge averages; null otherwise.
+  // w_lane / e_lane: the lanes on the panel's west / east cube edge (first / last strip)
+  T *fside;
+  bool w_lane, e_lane;
+  int N;
   // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6).
   // The step-invariant row inputs (metric, 1/g, winds) are loaded straight into the slot their
   // row will hold: with PF = 2 the loads of row y+2 are issued at the end of iteration y, when
@@ -170,6 +176,13 @@ struct March {
             st_at(Qo, (uint32_t)((y - 4) * ld + x), out);
             part -= (double)flx;               // used with MF-PR only
           }
+          if (fside) {   // MF-AF: cube-edge fluxes of output row r = y - 4 (f2: left edge of the lane's cell)
+            const int r = y - 4;
+            if (w_lane) st_at(fside, (uint32_t)(1 * N + r - NG), f2);
+            if (e_lane) st_at(fside, (uint32_t)(0 * N + r - NG), f2);
+            if (out_lane && r == NG) st_at(fside, (uint32_t)(3 * N + x - NG), g2prev);        // edge j0
+            if (out_lane && r == iend - 1) st_at(fside, (uint32_t)(2 * N + x - NG), g2);      // edge jend
+          }
         }
         g1prev = g1;
         g2prev = g2;
@@ -264,6 +277,10 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.ap1 = ((lane + 1) & 63) << 2;
   m.ap2 = ((lane + 2) & 63) << 2;
   m.out_lane = lane >= 3 && lane < 3 + nx;
+  m.N = N;
+  m.w_lane = ti == NG && lane == 3;
+  m.e_lane = ti + nx == iend && lane == 3 + nx;
+  m.fside = D.mf == 2 ? reinterpret_cast<T *>(D.work) + (int64_t)(tr * 6 + p) * 4 * N : nullptr;
   m.need_col = m.x >= 1 && m.x < iend + 3;   // columns the stencils reach
   m.dtdx = T(D.dt / D.dx);
   m.dtdy = T(D.dt / D.dy);

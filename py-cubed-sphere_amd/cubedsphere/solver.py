@@ -220,20 +220,23 @@ class Advection:
         # FP32 with the fused step: the state is stored from a per-tracer constant (the initial
         # field's minimum), Q = S + C gfix + offset, so that on a flat background S is ~0 and
         # increments far below an FP32 ulp of Q there are kept instead of rounded away
-        general = self.opts["et"] != 3 or self.opts["mf"] == 2
+        general = self.opts["et"] != 3          # ET-S72 / ET-PL07: the library's general step
         self.offset = torch.zeros(ntr, **f64) if (self.dtype == "f32" and not general) else None
         nparts = self.lib.csadv_partials_per_tracer(N, self.tile)
         _lib.check(nparts, None, "csadv_partials_per_tracer")
         self.partials = torch.zeros(ntr * nparts, **f64)
         self.ctl = torch.zeros(4, device=dev, dtype=torch.int32)
-        # ET-S72, ET-PL07 and MF-AF run the library's general step, which needs a work buffer
+        # ET-S72 and ET-PL07 run the library's general step, which needs a work buffer; the fused
+        # step with MF-AF keeps the second pass's cube-edge fluxes there ([ntr][6][4][N])
         self.work = None
-        if self.opts["et"] != 3 or self.opts["mf"] == 2:
+        if general:
             nbytes = ctypes.c_int64()
             code = _lib.load().csadv_work_bytes(N, _lib.CSADV_F64 if self.dtype == "f64" else _lib.CSADV_F32,
                                                 ctypes.byref(nbytes))
             _lib.check(code, None, "csadv_work_bytes")
             self.work = torch.zeros(int(nbytes.value), device=dev, dtype=torch.uint8)
+        elif self.opts["mf"] == 2:
+            self.work = torch.zeros(ntr * 24 * N, device=dev, dtype=td)
 
     def device_bytes(self):
         """Device memory held by this solve's own buffers (tables shared per N excluded)."""
