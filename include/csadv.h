@@ -226,6 +226,15 @@ int csadv_profile_read(csadv_t *h, double *total_ms, int32_t *launches);
  * reference: measurement of the kernel's roofline. */
 int csadv_time_step_kernel(csadv_t *h, int32_t reps, void *stream, double *us_per_launch);
 
+/* Mean duration in microseconds of the step kernel over reps rounds of one launch per handle
+ * (handles[0 .. nhandles-1], reps * nhandles <= 8192), replayed from one HIP graph between one
+ * event pair on `stream`.  With handles over distinct state buffers whose total exceeds the
+ * last-level cache (256 MB on MI355X), each launch streams its state from HBM, where
+ * csadv_time_step_kernel's repeats of one step find it in cache.  Each launch redoes its handle's
+ * current step.  Fused step only.  No counterpart in the reference: measurement. */
+int csadv_time_step_kernel_rotating(csadv_t *const *handles, int32_t nhandles, int32_t reps, void *stream,
+                                    double *us_per_launch);
+
 const char *csadv_last_error(csadv_t *h);
 void csadv_destroy(csadv_t *h);
 int csadv_abi_version(void);

@@ -1342,6 +1342,52 @@ int csadv_time_step_kernel(csadv_t *h, int32_t reps, void *stream, double *us_pe
   return CSADV_OK;
 }
 
+// Mean duration of the step kernel over reps rounds of one launch per handle, all captured in one
+// graph between one event pair (measurement only): with handles over distinct state buffers
+// whose total exceeds the last-level cache, every launch streams its state from HBM.  Each
+// launch redoes its handle's current step, as in csadv_time_step_kernel.
+int csadv_time_step_kernel_rotating(csadv_t *const *handles, int32_t nhandles, int32_t reps, void *stream,
+                                    double *us_per_launch) {
+  if (!handles || nhandles < 1 || !us_per_launch || reps < 1 || (int64_t)reps * nhandles > 8192) return CSADV_E_ARG;
+  csadv_t *h0 = handles[0];
+  if (!h0) return CSADV_E_ARG;
+  for (int k = 0; k < nhandles; ++k) {
+    if (!handles[k] || !handles[k]->dev) return fail(h0, CSADV_E_STATE, "rotating timing: a handle is not created");
+    if (handles[k]->general) return fail(h0, CSADV_E_STATE, "the general step has no single step kernel to time");
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (!h0->cap) HIPCHK(h0, hipStreamCreateWithFlags(&h0->cap, hipStreamNonBlocking));
+  HIPCHK(h0, hipStreamBeginCapture(h0->cap, hipStreamCaptureModeRelaxed));
+  hipError_t e = hipSuccess;
+  for (int r = 0; r < reps && e == hipSuccess; ++r)
+    for (int k = 0; k < nhandles && e == hipSuccess; ++k)
+      e = handles[k]->dtype == CSADV_F64 ? launch_step_dt<double>(handles[k], h0->cap)
+                                         : launch_step_dt<float>(handles[k], h0->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h0->cap, &g);
+  if (e != hipSuccess || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(h0, CSADV_E_HIP, std::string("timing capture: ") + hipGetErrorString(e != hipSuccess ? e : ec));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(h0, CSADV_E_HIP, std::string("timing graph: ") + hipGetErrorString(e));
+  hipEvent_t a = nullptr, b = nullptr;
+  float ms = 0.f;
+  if ((e = hipEventCreate(&a)) == hipSuccess && (e = hipEventCreate(&b)) == hipSuccess &&
+      (e = hipGraphLaunch(x, st)) == hipSuccess &&                          // warm-up replay
+      (e = hipEventRecord(a, st)) == hipSuccess && (e = hipGraphLaunch(x, st)) == hipSuccess &&
+      (e = hipEventRecord(b, st)) == hipSuccess && (e = hipEventSynchronize(b)) == hipSuccess)
+    e = hipEventElapsedTime(&ms, a, b);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  (void)hipGraphExecDestroy(x);
+  if (e != hipSuccess) return fail(h0, CSADV_E_HIP, std::string("timing run: ") + hipGetErrorString(e));
+  *us_per_launch = 1e3 * (double)ms / ((double)reps * nhandles);
+  return CSADV_OK;
+}
+
 const char *csadv_last_error(csadv_t *h) { return h ? h->err.c_str() : "null handle"; }
 
 void csadv_destroy(csadv_t *h) {

@@ -18,7 +18,7 @@ SYMBOLS = (
     "csadv_create", "csadv_partials_per_tracer", "csadv_work_bytes", "csadv_tile_rows", "csadv_update", "csadv_wind_init",
     "csadv_ghost_fill", "csadv_step", "csadv_set_graph", "csadv_finalize", "csadv_mass", "csadv_norms",
     "csadv_diagnostics", "csadv_diagnostics_noalloc", "csadv_status",
-    "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_time_step_kernel", "csadv_remap_latlon", "csadv_last_error", "csadv_destroy",
+    "csadv_current", "csadv_profile", "csadv_profile_read", "csadv_time_step_kernel", "csadv_time_step_kernel_rotating", "csadv_remap_latlon", "csadv_last_error", "csadv_destroy",
     "csadv_abi_version",
 )
 
@@ -82,6 +82,7 @@ def load():
     lib.csadv_profile.argtypes = [vp, i32]
     lib.csadv_profile_read.argtypes = [vp, dbl, ctypes.POINTER(ctypes.c_int32)]
     lib.csadv_time_step_kernel.argtypes = [vp, i32, vp, dbl]
+    lib.csadv_time_step_kernel_rotating.argtypes = [ctypes.POINTER(vp), i32, i32, vp, dbl]
     lib.csadv_remap_latlon.argtypes = [vp, i32, i32, vp, i32, i32, vp, vp, vp, vp, vp]
     lib.csadv_last_error.argtypes = [vp]
     lib.csadv_last_error.restype = ctypes.c_char_p

@@ -59,6 +59,45 @@ def tune_batch(specs, dtype="f32", candidates=None, steps=32, warmup=16, device=
     return best, timings
 
 
+def tune_batch_per_n(specs, dtype="f32", candidates=None, steps=32, warmup=16, device="cuda:0", graph_steps=16,
+                     gain=0.01, **opts):
+    """A tile per N for the batch ``specs`` = [(N, case, nsteps)]: the best uniform tile first,
+    then a coordinate descent over the sizes (most cells first), each trying every other
+    candidate for its solves with the others fixed and keeping a change that is faster by more
+    than ``gain``.  One batch is built; tiles change through the descriptors.  Returns
+    ({N: tile}, {trial: seconds per step})."""
+    cands = list(candidates) if candidates is not None else tile_ids()
+    need = warmup + steps
+    Ns = sorted({N for N, _, _ in specs})
+    members = make_members([(N, case, max(n, need)) for N, case, n in specs], dtype=dtype, device=device,
+                           tiles={N: cands[0] for N in Ns}, **opts)
+    b = Batch(members, graph_steps=graph_steps).init()
+    timings = {}
+
+    def measure(tiles):
+        b.set_tiles(tiles)
+        sec = _time_steps(b, steps, warmup)
+        timings[",".join("%d:%d" % (N, tiles[N]) for N in Ns)] = sec
+        return sec
+
+    uniform = {t: measure({N: t for N in Ns}) for t in cands}
+    t0 = min(uniform, key=uniform.get)
+    best, cur = {N: t0 for N in Ns}, uniform[t0]
+    work = {N: sum(6 * N * N * case.ntr for n_, case, _ in specs if n_ == N) for N in Ns}
+    for N in sorted(Ns, key=lambda n: -work[n]):
+        for t in cands:
+            if t == best[N]:
+                continue
+            trial = dict(best)
+            trial[N] = t
+            sec = measure(trial)
+            if sec < cur * (1.0 - gain):
+                best, cur = trial, sec
+    b.close()
+    del members
+    return best, timings
+
+
 def tune_solve(N, dtype="f32", ntr=1, candidates=None, steps=32, warmup=16, device="cuda:0", seed=0, **opts):
     """Best tile of one solve of size N (a seeded random solid-body case with ntr tracers)."""
     case = random_solid_body(np.random.default_rng(seed), ntr)

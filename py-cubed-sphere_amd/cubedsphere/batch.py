@@ -104,6 +104,14 @@ class Batch:
         _lib.check(self.lib.csadv_update(self.handle, descs, len(self.members), self._stream()), self.handle,
                    "csadv_update")
 
+    def set_tiles(self, tiles):
+        """Tile id per N ({N: tile}) from the next step on: the pending mass-fixer sums are folded
+        with the tiles they were written with, then the handle takes the new ones."""
+        _lib.check(self.lib.csadv_finalize(self.handle, -1, self._stream()), self.handle, "csadv_finalize")
+        for m in self.members:
+            m.set_tile(tiles.get(m.grid.N, m.tile))
+        self.refresh()
+
     def cell_updates_per_step(self):
         return sum(6 * m.grid.N ** 2 * m.case.ntr for m in self.members)
 

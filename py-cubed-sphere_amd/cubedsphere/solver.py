@@ -116,6 +116,19 @@ def correction_metric(rec, gflat, pstride):
     return gfix
 
 
+def time_step_kernel_rotating(solves, reps=20):
+    """Mean microseconds of the step kernel over ``reps`` rounds of one launch per solve (each
+    solve its own handle and state buffers): with the solves' states together larger than the
+    256 MB last-level cache, every launch streams its state from HBM."""
+    lib = _lib.load()
+    handles = (ctypes.c_void_p * len(solves))(*[s.handle for s in solves])
+    us = ctypes.c_double()
+    stream = ctypes.c_void_p(torch.cuda.current_stream(solves[0].device).cuda_stream)
+    _lib.check(lib.csadv_time_step_kernel_rotating(handles, len(solves), int(reps), stream, ctypes.byref(us)),
+               solves[0].handle, "csadv_time_step_kernel_rotating")
+    return us.value
+
+
 _TABLES = {}
 
 
@@ -222,8 +235,12 @@ class Advection:
         # increments far below an FP32 ulp of Q there are kept instead of rounded away
         general = self.opts["et"] != 3          # ET-S72 / ET-PL07: the library's general step
         self.offset = torch.zeros(ntr, **f64) if (self.dtype == "f32" and not general) else None
-        nparts = self.lib.csadv_partials_per_tracer(N, self.tile)
-        _lib.check(nparts, None, "csadv_partials_per_tracer")
+        # partial sums for the tile of most wavefronts, so that the tile may change (set_tile)
+        nparts = 0
+        t = 0
+        while self.lib.csadv_tile_rows(t) > 0:
+            nparts = max(nparts, _lib.check(self.lib.csadv_partials_per_tracer(N, t), None, "csadv_partials_per_tracer"))
+            t += 1
         self.partials = torch.zeros(ntr * nparts, **f64)
         self.ctl = torch.zeros(4, device=dev, dtype=torch.int32)
         # ET-S72 and ET-PL07 run the library's general step, which needs a work buffer; the fused
@@ -369,6 +386,11 @@ class Advection:
         if not np.array_equal(have, row):
             self.coef[n * row.size:(n + 1) * row.size].copy_(torch.from_numpy(row))
             rows[n] = row
+
+    def set_tile(self, tile):
+        """Use tile id ``tile`` from the next step on (the handle needs refresh(); the pending
+        mass-fixer partials must be folded first, as Batch.set_tiles does)."""
+        self.tile = int(tile)
 
     def refresh(self):
         """Hand changed buffer pointers to the library (the handle's descriptor copy)."""

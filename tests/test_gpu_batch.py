@@ -146,3 +146,33 @@ def test_fp32_mass_fixer_keeps_mass_and_tracks_fp64():
         l2[mf] = np.linalg.norm(out["f32"] - out["f64"]) / np.linalg.norm(out["f64"])
     assert l2[3] <= 1.5e-6 and l2[3] <= 1.5 * l2[1] + 1e-7, l2
     assert drift <= 2e-7, drift
+
+
+def test_tiles_changed_mid_run_keep_oracle_parity():
+    """Batch.set_tiles between steps (the per-N autotuner's move): the pending mass-fixer sums
+    are folded with the tiles they were written with, so the run stays on the oracle."""
+    from cubedsphere.batch import Batch, make_members
+    nsteps = 8
+    specs = _specs(nsteps)
+    b = Batch(make_members(specs, dtype="f64"), graph_steps=0).init()
+    b.step(3)
+    b.set_tiles({16: 7, 24: 3, 20: 1, 32: 11})
+    b.step(3)
+    b.set_tiles({16: 0, 24: 0, 20: 0, 32: 0})
+    b.step(2)
+    for i in (0, 1, 3):
+        m = b.members[i]
+        grid, sim = run_oracle(m.grid.N, m.case, m.dt, nsteps)
+        got = m.state()
+        refs = sim.Qs if hasattr(sim, "Qs") else [sim.Q]
+        for n, ref in enumerate(refs):
+            assert rel_max(got[n], interior_dev(grid, ref)) < 1e-12, (i, n)
+
+
+def test_per_n_autotuner_returns_a_tile_per_size():
+    from cubedsphere.autotune import tile_ids, tune_batch_per_n
+    specs = _specs(4)
+    tiles, timings = tune_batch_per_n(specs, dtype="f32", candidates=[0, 1, 11], steps=4, warmup=2)
+    assert set(tiles) == {16, 24, 20, 32} and all(t in (0, 1, 11) for t in tiles.values())
+    assert len(timings) >= 3 and all(v > 0 for v in timings.values())
+    assert len(tile_ids()) == 12
