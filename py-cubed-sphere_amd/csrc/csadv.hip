@@ -330,25 +330,9 @@ __global__ void k_clear_corr(Solve *solves, int first) {
 
 // ------------------------------------------------------------------ MF-AF on the fused step
 // The fused step updates every cell with its own second-pass fluxes and, with MF-AF, keeps the
-// fluxes through the 24 panel sides in work ([tracer][panel][E, W, N, S][N], scaled by dt/dx as
-// the step forms them).  MF-AF (src/edges_treatment.py:231-278) replaces each by the average of
-// the two panels' fluxes through that cube edge, 0.5 f + 0.5 (sign f'), with the neighbour's side
-// and sign from the halo orientation table; this kernel adds the difference to the boundary
-// cells of the new state: Q += (f - f_avg) / g at the east and north sides (their fluxes leave
-// the cell there) and (f_avg - f) / g at the west and south sides.  One thread per boundary
-// cell of a panel (4 (N - 1) of them), so a corner cell takes both of its sides' differences in
-// one fixed order.  grid: x = boundary cells of the 6 panels, y = solve, z = tracer.
-template <typename T>
-__device__ __forceinline__ T af_average(const T *F, const int32_t *orient, int N, int p, int side, int k) {
-  const T own = F[(p * 4 + side) * N + k];
-  const int32_t *o = orient + (p * 4 + side) * 7;
-  int q, i, j, sq, sign;
-  halo_src(o, 0, k + NG, q, i, j);
-  across(o, side, sq, sign);
-  const T nb = F[(q * 4 + sq) * N + (sq < 2 ? j : i) - NG];
-  return T(0.5) * own + T(0.5) * (sign > 0 ? nb : -nb);
-}
-
+// fluxes through the 24 panel sides in work ([tracer][panel][E, W, N, S][N]); this kernel
+// replaces them by the cube-edge averages in the boundary cells of the new state (af_fix_cell).
+// grid: x = boundary cells of the 6 panels, y = solve, z = tracer.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_af_fix(const Solve *solves) {
   const Solve &S = solves[blockIdx.y];
@@ -357,26 +341,7 @@ __global__ __launch_bounds__(BLOCK) void k_af_fix(const Solve *solves) {
   if (D.mf != 2 || tr >= D.ntr) return;
   const int run = D.ctl[2];                 // the step k_ghost started (< 0: the solve is halted)
   if (run < 0 || (D.stop > 0 && run >= D.stop)) return;
-  const int N = D.N, ld = D.ld, i0 = NG, iend = N + NG;
-  const int ring = 4 * (N - 1);
-  const int idx = blockIdx.x * BLOCK + threadIdx.x;
-  if (idx >= 6 * ring) return;
-  const int p = idx / ring, c = idx - p * ring;
-  // ring order: south row, north row, west column, east column (corners in the rows)
-  int i, j;
-  if (c < N) { i = i0 + c; j = i0; }
-  else if (c < 2 * N) { i = i0 + c - N; j = iend - 1; }
-  else if (c < 3 * N - 2) { i = i0; j = i0 + 1 + c - 2 * N; }
-  else { i = iend - 1; j = i0 + 1 + c - (3 * N - 2); }
-  const T *F = reinterpret_cast<const T *>(D.work) + (int64_t)tr * 24 * N;
-  T d = T(0);
-  if (i == iend - 1) d += F[(p * 4 + 0) * N + j - NG] - af_average(F, D.orient, N, p, 0, j - NG);
-  if (i == i0) d += af_average(F, D.orient, N, p, 1, j - NG) - F[(p * 4 + 1) * N + j - NG];
-  if (j == iend - 1) d += F[(p * 4 + 2) * N + i - NG] - af_average(F, D.orient, N, p, 2, i - NG);
-  if (j == i0) d += af_average(F, D.orient, N, p, 3, i - NG) - F[(p * 4 + 3) * N + i - NG];
-  T *Q = reinterpret_cast<T *>(((run + 1) & 1) ? D.q1 : D.q0) + tr * S.tstride + p * S.pstride;
-  const T rg = reinterpret_cast<const T *>(D.rgc)[(int64_t)j * ld + i];
-  Q[(int64_t)j * ld + i] += d * rg;
+  af_fix_cell<T>(S, tr, run, reinterpret_cast<const T *>(D.work) + (int64_t)tr * 24 * D.N, blockIdx.x * BLOCK + threadIdx.x);
 }
 
 // ------------------------------------------------------------------ lat-lon output remap
@@ -763,18 +728,8 @@ __global__ void k_gp_wind_rk2_copy(const Solve *solves, int s) {
   }
 }
 
-// the step is complete for every tracer: advance the step counter
-__global__ void k_gp_advance(const Solve *solves, int s) {
-  const csadv_solve_desc &D = solves[s].d;
-  if ((D.stop > 0 && D.ctl[1] >= D.stop) || D.ctl[0]) return;     // finished or halted
-  if (threadIdx.x == 0) D.ctl[1] = D.ctl[1] + 1;
-}
-
 }  // namespace gp
 
-void launch_gp_advance(csadv_t *h, hipStream_t st, int s) {
-  hipLaunchKernelGGL(gp::k_gp_advance, dim3(1), dim3(64), 0, st, h->dev, s);
-}
 
 
 int fail(csadv_t *h, int code, const std::string &msg) {

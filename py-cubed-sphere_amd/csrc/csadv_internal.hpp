@@ -63,9 +63,6 @@ inline int64_t diag_scratch_words(int ntracers) {
 // Full 4-layer duo-grid ghost fill of every tracer's current buffer (csadv.hip).
 int ghost_fill(csadv_t *h, hipStream_t st);
 
-// The general step's last kernel of a step: advance the solve's step counter (csadv.hip).
-void launch_gp_advance(csadv_t *h, hipStream_t st, int s);
-
 // The fused step kernel of every solve for reconstruction R (step_<dtype>_r<R>.hip).
 template <typename T, int R>
 hipError_t launch_step_r(csadv_t *h, hipStream_t st);

@@ -72,8 +72,8 @@ int check_compatible(const csadv_solve_desc *solves, int nsolve, std::string &ms
 }
 
 void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs) {
-  strips = (N + STRIP_MAX - 1) / STRIP_MAX;
-  strip_w = (N + strips - 1) / strips;          // balanced strips of at most 58 columns
+  strips = strip_count(N);
+  strip_w = (N + strips - 1) / strips;
   seg_l = std::min(kSegRows[tile], N);
   segs = (N + seg_l - 1) / seg_l;
 }
@@ -125,7 +125,7 @@ int work_bytes(int N, int dtype, int64_t *bytes) {
   if (!bytes || N < 8 || N > 8192 || (dtype != CSADV_F64 && dtype != CSADV_F32)) return CSADV_E_ARG;
   const WorkLayout w = work_layout(N, general_ld(N), general_ldu(N));
   const int64_t s = dtype == CSADV_F64 ? 8 : 4;
-  *bytes = w.total * s + (int64_t)kRedGen * 8;
+  *bytes = w.total * s + w.n_red * 8;
   return CSADV_OK;
 }
 

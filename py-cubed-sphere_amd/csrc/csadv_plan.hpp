@@ -23,7 +23,6 @@ constexpr int BLOCK = 256;           // threads per block of the whole-array ker
 constexpr int WAVE = 64;             // CDNA4 wavefront
 constexpr int kXcds = 8;             // MI355X: 8 XCDs; workgroup b is dispatched to XCD b % 8
 constexpr int STRIP_MAX = WAVE - 6;  // interior columns of one wavefront (3-cell halo each side)
-constexpr int kRedGen = 512;         // blocks of the general step's divergence reduction
 constexpr int kDiagBlocks = 256;     // blocks of a diagnostics reduction (partials per tracer)
 constexpr int kDiagWords = 8;        // doubles per diagnostics partial
 constexpr int kMaxTracers = 64;      // tracers per solve
@@ -50,34 +49,48 @@ struct Item {
   int32_t x, y, z, w;
 };
 
-// General step: element counts of one solve's work buffer (dtype T), one tracer at a time.
+// The general step's work buffer of one solve (dtype T), used by one tracer at a time:
+// element offsets of the first pass's Qx and Qy and, with MF-PR, the second pass's div (cell
+// arrays [6][M][ld]), the second pass's cube-edge fluxes with MF-AF ([6][4][N]), then a tail
+// of n_red doubles: the MF-PR partial sums of the wavefronts, and their total in the last one.
 struct WorkLayout {
-  int64_t cell, xe, ye, strips;          // element counts
-  int64_t o_qx, o_qy, o_pxl, o_pxr, o_pyl, o_pyr, o_fx, o_fy, o_div, o_strip, o_avg, o_red;
-  int64_t total;                         // elements of T (+ the FP64 reduction tail)
+  int64_t cell;                          // elements of one cell array
+  int64_t o_qx, o_qy, o_div, o_side;     // element offsets
+  int64_t o_red, n_red;                  // offset (in T units, 8-aligned) and doubles of the tail
+  int64_t total;                         // elements of T before the tail (= o_red)
 };
 
+// Column strips of a panel of N columns: the fewest balanced strips (ceil(N / strips) columns,
+// the last one the rest) of at most STRIP_MAX columns whose last strip keeps at least
+// kMinLastStrip columns (a stencil beside the panel's east side reaches five columns back).
+constexpr int kMinLastStrip = 4;
+CSADV_HD inline int strip_count(int N) {
+  int strips = (N + STRIP_MAX - 1) / STRIP_MAX;
+  while (N - (strips - 1) * ((N + strips - 1) / strips) < kMinLastStrip && (N + strips - 1) / strips > 1) ++strips;
+  return strips;
+}
+
+// Most wavefronts of one tracer's pass at size N over every tile shape (the shortest segment).
+CSADV_HD inline int64_t general_max_items(int N) {
+  int rows = kSegRows[0];
+  for (int k = 1; k < kNumTiles; ++k) rows = kSegRows[k] < rows ? kSegRows[k] : rows;
+  const int64_t strips = strip_count(N), segs = (N + rows - 1) / rows;
+  return 6 * strips * segs;
+}
+
 CSADV_HD inline WorkLayout work_layout(int N, int ld, int ldu) {
+  (void)ldu;
   WorkLayout w{};
   const int M = N + 2 * NG;
   w.cell = (int64_t)6 * M * ld;
-  w.xe = (int64_t)6 * M * ldu;
-  w.ye = (int64_t)6 * (M + 1) * ld;
-  w.strips = (int64_t)6 * 4 * 4 * M;
   int64_t o = 0;
   w.o_qx = o; o += w.cell;
   w.o_qy = o; o += w.cell;
-  w.o_pxl = o; o += w.cell;
-  w.o_pxr = o; o += w.cell;
-  w.o_pyl = o; o += w.cell;
-  w.o_pyr = o; o += w.cell;
-  w.o_fx = o; o += w.xe;
-  w.o_fy = o; o += w.ye;
   w.o_div = o; o += w.cell;
-  w.o_strip = o; o += 2 * w.strips;      // copy-halo strips (two fields)
-  w.o_avg = o; o += (int64_t)6 * 4 * M;  // cube-edge averages
+  w.o_side = o; o += (int64_t)6 * 4 * N;
   o = (o + 7) / 8 * 8;
-  w.o_red = o;                           // FP64 partial sums, kRedGen doubles (in T units)
+  w.o_red = o;
+  w.n_red = general_max_items(N) + 1;
   w.total = o;
   return w;
 }

@@ -331,24 +331,6 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   }
 }
 
-// Bring every cache line of the kernel-argument descriptor into the scalar cache with loads
-// issued together: an empty asm takes one word of each line and passes v through unchanged, so
-// v exists only once all of them have arrived (one scalar round trip); the field loads at their
-// first use then hit the cache instead of waiting for their lines one after another (the
-// one-solve step kernel's start took four such dependent round trips).  The asm is neither
-// volatile nor a memory clobber, so the loads after it may still be scalar loads.  (For a
-// descriptor in global memory the compiler reads the words with vector loads, which warm no
-// scalar cache: the multi-solve kernel does not use it.)
-__device__ __forceinline__ int touch_solve(const Solve &S, int v) {
-  static_assert(sizeof(Solve) == 432, "one word in each 64-byte line of the descriptor");
-  const char *b = reinterpret_cast<const char *>(&S);
-  int w[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) __builtin_memcpy(&w[k], b + (k < 7 ? 64 * k : 428), 4);
-  asm("" : "=s"(v) : "0"(v), "s"(w[0]), "s"(w[1]), "s"(w[2]), "s"(w[3]), "s"(w[4]), "s"(w[5]), "s"(w[6]), "s"(w[7]));
-  return v;
-}
-
 template <typename T, int RECON, bool RK2, int SPLIT, bool WIND>
 __global__ __launch_bounds__(WAVE) void k_step(const Solve *__restrict__ solves, const int4 *__restrict__ work) {
   const int4 wk = work[blockIdx.x];

@@ -116,12 +116,13 @@ int main() {
     cases += 3;
   }
   // ---- tile shapes cover every panel for all N and tiles
-  for (int N = 8; N <= 8192; N += (N < 600 ? 1 : 97)) {
+  for (int N = 8; N <= 8192; ++N) {
     for (int t = 0; t < kNumTiles; ++t) {
       int sw, ns, sl, nsg;
       tiling(N, t, sw, ns, sl, nsg);
       CHECK(sw >= 1 && sw <= STRIP_MAX && (int64_t)sw * ns >= N && (int64_t)sw * (ns - 1) < N,
             "strips N=%d tile=%d: %d x %d", N, t, ns, sw);
+      CHECK(N - sw * (ns - 1) >= kMinLastStrip, "last strip N=%d: %d columns", N, N - sw * (ns - 1));
       CHECK(sl >= 1 && (int64_t)sl * nsg >= N && (int64_t)sl * (nsg - 1) < N, "segments N=%d tile=%d", N, t);
       CHECK(partials_per_tracer(N, t) == 6 * ns * nsg, "partials N=%d", N);
       ++cases;
@@ -169,14 +170,15 @@ int main() {
   // ---- general step layout: arrays disjoint, in bounds, FP64 tail aligned
   for (int N : {8, 9, 17, 48, 384, 768, 8192}) {
     const WorkLayout w = work_layout(N, general_ld(N), general_ldu(N));
-    const int64_t offs[] = {w.o_qx, w.o_qy, w.o_pxl, w.o_pxr, w.o_pyl, w.o_pyr, w.o_fx, w.o_fy, w.o_div, w.o_strip, w.o_avg, w.o_red};
-    const int64_t lens[] = {w.cell, w.cell, w.cell, w.cell, w.cell, w.cell, w.xe, w.ye, w.cell, 2 * w.strips,
-                            (int64_t)24 * (N + 2 * NG), 0};
-    for (int k = 0; k + 1 < 12; ++k) CHECK(offs[k] + lens[k] <= offs[k + 1], "layout overlap N=%d array %d", N, k);
+    const int64_t offs[] = {w.o_qx, w.o_qy, w.o_div, w.o_side, w.o_red};
+    const int64_t lens[] = {w.cell, w.cell, w.cell, (int64_t)24 * N};
+    for (int k = 0; k + 1 < 5; ++k) CHECK(offs[k] + lens[k] <= offs[k + 1], "layout overlap N=%d array %d", N, k);
     CHECK(w.o_red % 8 == 0 && w.total == w.o_red, "reduction tail N=%d", N);
+    // the tail holds every wavefront's partial of any tile shape, and their total
+    for (int t = 0; t < kNumTiles; ++t) CHECK(partials_per_tracer(N, t) < w.n_red, "partials N=%d tile %d", N, t);
     int64_t f64 = 0, f32 = 0;
     CHECK(work_bytes(N, CSADV_F64, &f64) == CSADV_OK && work_bytes(N, CSADV_F32, &f32) == CSADV_OK, "work_bytes");
-    CHECK(f64 == w.total * 8 + kRedGen * 8 && f32 == w.total * 4 + kRedGen * 8, "work_bytes N=%d", N);
+    CHECK(f64 == w.total * 8 + w.n_red * 8 && f32 == w.total * 4 + w.n_red * 8, "work_bytes N=%d", N);
     ++cases;
   }
   int64_t dummy;

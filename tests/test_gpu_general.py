@@ -81,6 +81,24 @@ def test_general_step_multitracer_and_conservation():
     assert abs(sol.mass() - m0) <= 1e-13 * abs(m0)
 
 
+@pytest.mark.parametrize("opts", [dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),
+                                  dict(recon=2, opsplit=1, et=1, mt=1, dp=2, mf=3),
+                                  dict(recon=4, opsplit=2, et=2, mt=1, dp=1, mf=2)],
+                         ids=["etpl07", "ets72-rk2-mfpr", "etpl07-mfaf"])
+def test_general_step_four_tracers(opts):
+    """Four tracers through the general step (passes run tracer by tracer and the last kernel of
+    the last tracer ends the step): every tracer against the oracle."""
+    from cubedsphere.cases import workload
+    N, dt, nsteps = 24, 0.004, 7
+    case = workload("nl10_deformation")
+    sol = _solve(N, case, dt, nsteps, **opts)
+    grid, sim = run_oracle(N, case, dt, nsteps, **opts)
+    got = sol.state()
+    for n in range(case.ntr):
+        assert rel_max(got[n], interior_dev(grid, sim.Qs[n])) < TOL64, n
+    assert sol.status() == (nsteps, False)
+
+
 def test_general_step_graph_replay_bitwise():
     from cubedsphere.cases import reference_case
     from cubedsphere.solver import Advection
