@@ -23,10 +23,12 @@ the next step.  The output side (``output_adv``) computes the errors, mass, the 
 remaps and the reference's data files; it draws no plots (cartopy is not available).  Errors
 are those of src/errors.py:99-113 at the final step.
 """
+import os
+
 import numpy as np
 import torch
 
-from . import config, seam
+from . import config, gridfile, seam
 from ._lib import CsadvError
 from .cases import reference_case
 from .grid import Grid
@@ -254,7 +256,8 @@ def main(pardir="par"):
     ap = config.get_advection_parameters(pardir)
     if gp.test_case != 5:
         raise CsadvError("only test case 5 (advection) is on this path; par file asks for %d" % gp.test_case)
-    g = Grid(gp.N)
+    g = gridfile.cubed_sphere(gp.N, gp.gridload, griddir=os.path.join(os.path.dirname(os.path.abspath(pardir)), "grid"),
+                              verbose=gp.showonscreen)
     sim = adv_simulation_par(g, ap.dt, ap.Tf, ap.ic, ap.vf, ap.tc, ap.recon, ap.dp, ap.opsplit, ap.et, ap.mt, ap.mf)
     if ap.tc == 1:
         return adv_sphere(g, sim, verbose=True)
