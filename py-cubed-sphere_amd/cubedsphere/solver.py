@@ -103,12 +103,11 @@ def correction_metric(rec, gflat, pstride):
         gw = np.sum(rec[kind + "_w"] * gflat[rec[kind + "_gsrc"]], axis=1)
         tgt = rec[kind + "_tgt"].astype(np.int64)
         loc = tgt % pstride
-        lo = np.full(pstride, np.inf)
-        hi = np.full(pstride, -np.inf)
-        np.minimum.at(lo, loc, gw)
-        np.maximum.at(hi, loc, gw)
-        seen = np.isfinite(lo)
-        spread = np.max(hi[seen] - lo[seen])
+        # spread of the values at each position over the panels (sorted groups of equal loc)
+        order = np.argsort(loc, kind="stable")
+        sl, sg = loc[order], gw[order]
+        starts = np.flatnonzero(np.r_[True, sl[1:] != sl[:-1]])
+        spread = np.max(np.maximum.reduceat(sg, starts) - np.minimum.reduceat(sg, starts))
         if spread > 1e-12 * np.max(np.abs(gflat)):
             raise RuntimeError("ghost metric weights differ between panels by %.3e" % spread)
         first = tgt < pstride

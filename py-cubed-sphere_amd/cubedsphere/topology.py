@@ -129,7 +129,8 @@ def ghost_recipes(grid, ld, depth=3):
       strip_tgt [n] int32, strip_src [n][4] int32, strip_w [n][4] f64
       corner_tgt [m] int32, corner_src [m][16] int32, corner_w [m][16] f64
     ``*_gsrc`` hold the offsets inside one panel (j * ld + i) for the metric of each source.
-    Corner stencils may read a neighbour strip's 4th layer, so the strip lookup covers depth 3.
+    Corner stencils may read a neighbour strip's 4th layer (depth 3), whose recipe is formed
+    where it is read.
     """
     N, M = grid.N, grid.M
     orient = orientation(N)
@@ -142,24 +143,45 @@ def ghost_recipes(grid, ld, depth=3):
     # addresses; when it runs down a column, the depth is fastest, so consecutive cells read
     # (and, on the east and west sides, write) the neighbouring cells of one row.
     tgt, src, wts, gsrc = [], [], [], []
-    strip_of = {}
+    iend = N + 4
+
+    orient_l = orient.tolist()
+    kmin_l = kmin.tolist()
+    w_l = w.tolist()
+
+    def strip_of(p, ti, tj):
+        """Recipe (source offsets, panel offsets, weights; Python lists) of the strip ghost at
+        (ti, tj) of panel p: the side and depth follow from where the cell lies (corners query
+        only strip cells with an interior along-edge index, up to depth 3)."""
+        if ti >= iend:
+            side, d, k = E, ti - iend, tj
+        elif ti < 4:
+            side, d, k = W, 3 - ti, tj
+        elif tj >= iend:
+            side, d, k = N_, tj - iend, ti
+        else:
+            side, d, k = S, 3 - tj, ti
+        q, ib, id_, ir, jb, jd, jr = orient_l[p][side]
+        k0 = kmin_l[d][k]
+        ss, gg = [], []
+        for l in range(4):
+            si, sj = ib + id_ * d + ir * (k0 + l), jb + jd * d + jr * (k0 + l)
+            ss.append(q * pstride + sj * ld + si)
+            gg.append(sj * ld + si)
+        return ss, gg, w_l[d][k]
+
     for p in range(6):
         for side in (E, W, N_, S):
             q, ib, id_, ir, jb, jd, jr = (int(v) for v in orient[p, side])
             parts = []
-            for d in range(4):   # corner stencils may reach the 4th layer of a strip
+            for d in range(depth):
                 K = kmin[d, ks][:, None] + np.arange(4)[None, :]
                 si = ib + id_ * d + ir * K
                 sj = jb + jd * d + jr * K
                 ti, tj = _ghost_target(side, d, ks, N)
                 ti = np.broadcast_to(ti, ks.shape)
                 tj = np.broadcast_to(tj, ks.shape)
-                s_off = q * pstride + sj * ld + si
-                g_off = sj * ld + si
-                for n in range(len(ks)):
-                    strip_of[(p, int(ti[n]), int(tj[n]))] = (s_off[n], g_off[n], w[d, ks[n], :])
-                if d < depth:
-                    parts.append((p * pstride + tj * ld + ti, s_off, g_off, w[d, ks, :]))
+                parts.append((p * pstride + tj * ld + ti, q * pstride + sj * ld + si, sj * ld + si, w[d, ks, :]))
             if ir != 0:
                 for t_, s_, g_, w_ in parts:
                     tgt.append(t_)
@@ -178,28 +200,28 @@ def ghost_recipes(grid, ld, depth=3):
         "strip_w": np.ascontiguousarray(np.concatenate(wts)),
     }
     # ---- corners (east / west halos, rows outside the panel)
-    iend = N + 4
     rows = [r for r in range(4 - depth, 4)] + [r for r in range(iend, iend + depth)]
     ctgt, csrc, cgsrc, cw = [], [], [], []
     for p in range(6):
         for side in (E, W):
-            q, ib, id_, ir, jb, jd, jr = (int(v) for v in orient[p, side])
+            q, ib, id_, ir, jb, jd, jr = orient_l[p][side]
             for d in range(depth):
                 for k in rows:
                     ti, tj = _ghost_target(side, d, k, N)
                     s_, g_, w_ = [], [], []
                     for l in range(4):
-                        r = int(kmin[d, k]) + l
+                        r = kmin_l[d][k] + l
                         si, sj = ib + id_ * d + ir * r, jb + jd * d + jr * r
+                        wl = w_l[d][k][l]
                         if 4 <= si < iend and 4 <= sj < iend:
                             s_.append(q * pstride + sj * ld + si)
                             g_.append(sj * ld + si)
-                            w_.append(w[d, k, l])
+                            w_.append(wl)
                         else:
-                            ss, gg, ww = strip_of[(q, si, sj)]
-                            s_.extend(int(v) for v in ss)
-                            g_.extend(int(v) for v in gg)
-                            w_.extend(float(w[d, k, l] * v) for v in ww)
+                            ss, gg, ww = strip_of(q, si, sj)
+                            s_.extend(ss)
+                            g_.extend(gg)
+                            w_.extend(wl * v for v in ww)
                     while len(s_) < 16:          # pad with zero-weight copies of the first source
                         s_.append(s_[0])
                         g_.append(g_[0])
