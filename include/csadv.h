@@ -48,8 +48,9 @@ enum csadv_status {
  *   mt    1 MT-0  | 2 MT-PL07   (valid pairs: split 1|2 with mt 1, split 3 with mt 2)
  *   mf    1 none  | 2 MF-AF     | 3 MF-PR
  * ET-DG runs the fused step kernel (with MF-AF: plus a fix-up of the cube-edge cells);
- * ET-S72 and ET-PL07 run the general step (a kernel per stage).  Both need the work buffer
- * where noted below.  All solves of one handle take the same one of the two.            */
+ * ET-S72 and ET-PL07 run the general step (two marched passes per tracer, the copy halo
+ * read from the adjacent panels in the passes).  Both need the work buffer where noted
+ * below.  All solves of one handle take the same one of the two.                          */
 typedef struct csadv_solve_desc {
   int32_t N, ntr, dtype;
   int32_t ld, ldu;                 /* row strides (elements) of cell / x-edge arrays   */

@@ -1,8 +1,9 @@
-"""The general step (ET-S72, ET-PL07, MF-AF) against the oracle.
+"""The general step (ET-S72, ET-PL07) and MF-AF against the oracle.
 
-These options run the library's stage-by-stage step (a kernel per stage of
-src/discrete_operators.py:18-101) instead of the fused step kernel.  Bar: FP64 within
-1e-12 relative of the oracle on identical inputs, as for the fused path.
+ET-S72 and ET-PL07 run the library's general step (two marched passes per tracer of
+src/discrete_operators.py:18-101, the copy halo read from the adjacent panels) instead of the
+fused step kernel; ET-DG with MF-AF runs the fused step with its cube-edge fix-up.  Bar: FP64
+within 1e-12 relative of the oracle on identical inputs, as for the fused path.
 """
 import numpy as np
 import pytest
