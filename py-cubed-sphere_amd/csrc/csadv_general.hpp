@@ -85,7 +85,9 @@ __device__ __forceinline__ Lines lane_lines(const csadv_solve_desc &D, int ps, i
 // cube-edge average of our and the neighbour panel's extrapolation.
 template <typename T> __device__ __forceinline__ T pl_ext(T c, T n) { return T(1.5) * c - T(0.5) * n; }
 template <typename T> __device__ __forceinline__ T pl_edge(T c0, T c1, T c2) {
-  return (T(3.0) * c0 + T(11.0) * c1 - T(2.0) * (c2 - c0)) / T(14.0);
+  // times the rounded 1/14 rather than divided by 14: within an ulp of the quotient, and a
+  // multiply where a division takes about ten instructions
+  return (T(3.0) * c0 + T(11.0) * c1 - T(2.0) * (c2 - c0)) * T(1.0 / 14.0);
 }
 template <typename T> __device__ __forceinline__ T pl_avg(T own, T nb) { return (own + nb) * T(0.5); }
 

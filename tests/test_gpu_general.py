@@ -113,9 +113,9 @@ def test_general_step_graph_replay_bitwise():
     assert np.array_equal(a.full_state(), b.full_state())
 
 
-# the smallest N the library takes (8: the three-cell extrapolation windows of opposite panel
-# sides and the flux predicate ec <= i0 + 1 || ec >= iend - 2 come closest) and odd N (a cell
-# centre on each pole)
+# the smallest N the library takes (8: the ET-PL07 cells beside opposite panel sides, i0-1 ..
+# i0+1 and iend-2 .. iend, come closest), odd N (a cell centre on each pole) and N = 9, whose
+# last row segment has one row
 SMALL_N_OPTS = [
     dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),        # ET-PL07, PL07 recon
     dict(recon=2, opsplit=1, et=2, mt=1, dp=1, mf=3),        # ET-PL07, CW84 recon, MF-PR
