@@ -43,7 +43,7 @@ class Completed:
     admitted: float
     done: float
     cell_updates: float
-    result: np.ndarray = field(repr=False)
+    result: np.ndarray = field(repr=False)     # [ntr][6][N][N] in the request's precision
     dt: float = 0.0         # time step the solve ran with (CFL 0.5 at t = 0)
 
     @property
@@ -196,7 +196,7 @@ class Server:
                 if not finished:
                     continue
                 for m in finished:
-                    res = m.state()
+                    res = m.state(native=True)
                     t = self.clock() - t0
                     req = m.request
                     c = Completed(req.rid, req.arrival, m.admitted, t, 6.0 * req.N ** 2 * req.case.ntr * req.nsteps, res,

@@ -435,12 +435,15 @@ class Advection:
     def finalize(self):
         _lib.check(self.lib.csadv_finalize(self.handle, self.index, self._stream()), self.handle, "csadv_finalize")
 
-    def state(self):
-        """Interior of every tracer after folding in pending corrections: [ntr][6][N][N] FP64 numpy."""
+    def state(self, native=False):
+        """Interior of every tracer after folding in pending corrections: [ntr][6][N][N] numpy,
+        FP64, or with ``native`` in the solve's own precision (the same values; an FP32 solve's
+        result then crosses PCIe at half the size)."""
         self.finalize()
         g = self.grid
         I = slice(g.i0, g.iend)
-        return self.q[self.current()][:, :, I, I].double().cpu().numpy()
+        q = self.q[self.current()][:, :, I, I]
+        return (q.contiguous() if native else q.double()).cpu().numpy()
 
     def full_state(self):
         """Every tracer including ghosts: [ntr][6][M][M] numpy (dtype of the solve)."""
