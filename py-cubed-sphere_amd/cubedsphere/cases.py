@@ -211,10 +211,12 @@ class Case:
         """Rows n = 0..nsteps-1: (s(t_n), s(t_{n-1})) with t_{-1} := t_0, the winds step n+1 uses
         (src/advection_timestep.py:31-37 with update_adv at :48-75)."""
         K = len(self.basis())
-        tab = np.zeros((max(nsteps, 1), 2, K))
-        for n in range(max(nsteps, 1)):
-            tab[n, 0] = self.coefficients(n * dt)
-            tab[n, 1] = self.coefficients(max(n - 1, 0) * dt)
+        n = max(nsteps, 1)
+        cur = np.array([self.coefficients(k * dt) for k in range(n)], dtype=np.float64).reshape(n, K)
+        tab = np.empty((n, 2, K))
+        tab[:, 0] = cur
+        tab[0, 1] = cur[0]              # s(t_{n-1}) is row n-1's s(t): the same value
+        tab[1:, 1] = cur[:-1]
         return tab
 
     def q(self, n, lon, lat, t):

@@ -35,6 +35,11 @@ PANEL_ROT = np.array([
 ], dtype=np.int64)
 
 
+# PANEL_ROT as (source component, sign) per output component, [6][3] each
+_ROT_IDX = np.array([[int(np.nonzero(row)[0][0]) for row in R] for R in PANEL_ROT], dtype=np.int64)
+_ROT_SGN = np.array([[float(row[np.nonzero(row)[0][0]]) for row in R] for R in PANEL_ROT])
+
+
 def _rotate(p, c0, c1, c2):
     """Apply the signed permutation of panel p exactly (no multiplications by +-1 rounding)."""
     comps = (c0, c1, c2)
@@ -73,6 +78,9 @@ class PointSet:
         ey0 = (-ty * fy, -tx * ty * fy, (1.0 + tx * tx) * fy)
         self.g = (1.0 + tx * tx) * (1.0 + ty * ty) / D3
         shape = (6,) + tuple(xx.shape)
+        if device is not None:
+            self._panels_device(c0, c1, c2, ex0, ey0, f64)
+            return
 
         def empty():
             return np.empty(shape) if device is None else torch.empty(shape, **f64)
@@ -109,6 +117,32 @@ class PointSet:
             self.ex_lat[p] = ex[0] * elat[0] + ex[1] * elat[1] + ex[2] * elat[2]
             self.ey_lon[p] = ey[0] * elon[0] + ey[1] * elon[1]
             self.ey_lat[p] = ey[0] * elat[0] + ey[1] * elat[1] + ey[2] * elat[2]
+        self.det = self.ex_lon * self.ey_lat - self.ey_lon * self.ex_lat
+
+    def _panels_device(self, c0, c1, c2, ex0, ey0, f64):
+        """The loop above for all six panels at once on the device: the signed permutations as a
+        gather and a product by +-1 (exact), then the same expressions elementwise over
+        [6][ny][nx], so every value has the bits of the per-panel construction."""
+        idx = torch.as_tensor(_ROT_IDX, device=f64["device"])
+        sgn = torch.as_tensor(_ROT_SGN, **f64)[:, :, None, None]
+        W = torch.stack([c0, c1, c2])[idx] * sgn                 # [6][3][ny][nx]
+        EX = torch.stack(list(ex0))[idx] * sgn
+        EY = torch.stack(list(ey0))[idx] * sgn
+        X, Y, Z = W[:, 0], W[:, 1], W[:, 2]
+        h = torch.hypot(X, Y)
+        lon = torch.atan2(Y, X)
+        lat = torch.atan2(Z, h)
+        pole = h == 0.0
+        sl = torch.where(pole, torch.sin(lon), Y / h)
+        cl = torch.where(pole, torch.cos(lon), X / h)
+        st, ct = Z, h
+        elon0, elon1 = -sl, cl
+        elat0, elat1, elat2 = -st * cl, -st * sl, ct
+        self.X, self.Y, self.Z, self.lon, self.lat = X, Y, Z, lon, lat
+        self.ex_lon = EX[:, 0] * elon0 + EX[:, 1] * elon1
+        self.ex_lat = EX[:, 0] * elat0 + EX[:, 1] * elat1 + EX[:, 2] * elat2
+        self.ey_lon = EY[:, 0] * elon0 + EY[:, 1] * elon1
+        self.ey_lat = EY[:, 0] * elat0 + EY[:, 1] * elat1 + EY[:, 2] * elat2
         self.det = self.ex_lon * self.ey_lat - self.ey_lon * self.ex_lat
 
 
