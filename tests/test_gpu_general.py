@@ -135,6 +135,23 @@ def test_general_step_small_and_odd_n(N, opts):
     assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
 
 
+@pytest.mark.parametrize("N", [61, 128])
+@pytest.mark.parametrize("opts", [dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),
+                                  dict(recon=4, opsplit=2, et=1, mt=1, dp=2, mf=2),
+                                  dict(recon=2, opsplit=1, et=2, mt=1, dp=1, mf=3)],
+                         ids=["etpl07", "ets72-rk2-mfaf", "etpl07-cw84-mfpr"])
+def test_general_step_several_column_strips(N, opts):
+    """N = 61 and 128 give two and three column strips per panel (at most 58 columns each) and
+    several row segments: strips away from the panel sides, which take no ET-PL07 x-edge
+    treatment, and segment boundaries inside the panel, against the oracle."""
+    from cubedsphere.cases import reference_case
+    dt, nsteps = 0.45 * (np.pi / 2 / N) / (2 * np.pi / 5 * 1.42), 8
+    case = reference_case(2, 1)
+    sol = _solve(N, case, dt, nsteps, **opts)
+    grid, sim = run_oracle(N, case, dt, nsteps, **opts)
+    assert rel_max(sol.state()[0], interior_dev(grid, sim.Q)) < TOL64
+
+
 @pytest.mark.parametrize("opts", [dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),
                                   dict(recon=3, opsplit=1, et=1, mt=1, dp=1, mf=3),
                                   dict(recon=3, opsplit=1, et=3, mt=1, dp=2, mf=2)],
