@@ -153,6 +153,31 @@ def test_general_step_several_column_strips(N, opts):
 
 
 @pytest.mark.parametrize("opts", [dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),
+                                  dict(recon=1, opsplit=1, et=1, mt=1, dp=2, mf=3)],
+                         ids=["etpl07", "ets72-rk2-mfpr"])
+def test_general_step_every_tile(opts):
+    """Every tile shape (4 to 32 rows per segment: one-row last segments, segments that start at
+    the ET-PL07 rows near the panel sides) against one oracle run, N = 61."""
+    from cubedsphere.cases import reference_case
+    from cubedsphere.solver import Advection
+    from cubedsphere import _lib
+    N = 61
+    dt, nsteps = 0.45 * (np.pi / 2 / N) / (2 * np.pi / 5 * 1.42), 5
+    case = reference_case(2, 1)
+    grid, sim = run_oracle(N, case, dt, nsteps, **opts)
+    ref = interior_dev(grid, sim.Q)
+    lib = _lib.load()
+    tile = 0
+    while lib.csadv_tile_rows(tile) > 0:
+        sol = Advection(N, case, dt, device="cuda:0", max_steps=nsteps, tile=tile, **opts).init_vars_adv()
+        sol.step(nsteps)
+        assert rel_max(sol.state()[0], ref) < TOL64, tile
+        sol.close()
+        tile += 1
+    assert tile == 12
+
+
+@pytest.mark.parametrize("opts", [dict(recon=3, opsplit=3, et=2, mt=2, dp=1, mf=1),
                                   dict(recon=3, opsplit=1, et=1, mt=1, dp=1, mf=3),
                                   dict(recon=3, opsplit=1, et=3, mt=1, dp=2, mf=2)],
                          ids=["etpl07", "ets72-mfpr", "rk2-mfaf"])
