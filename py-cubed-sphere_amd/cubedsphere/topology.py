@@ -161,6 +161,8 @@ def ghost_recipes(grid, ld, depth=3):
             side, d, k = N_, tj - iend, ti
         else:
             side, d, k = S, 3 - tj, ti
+        if not (0 <= d < 4 and 4 <= k < iend):
+            raise ValueError("(%d, %d) of panel %d is not a strip ghost cell" % (ti, tj, p))
         q, ib, id_, ir, jb, jd, jr = orient_l[p][side]
         k0 = kmin_l[d][k]
         ss, gg = [], []
