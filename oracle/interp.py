@@ -5,6 +5,14 @@ Restates src/interpolation.py:154-532:
   ghost_cells_adjacent_panels            :320-340  (ET-S72 / ET-PL07 copy halo)
   wind_edges2center_cubic_interpolation  :347-430
   wind_center2ghostedge_cubic_interpolation :436-532
+
+Pins: the edge strips through the reference's own Lagrange tables and halo strips (golden
+vectors), exactness for cubics along the neighbour lines and 4th-order convergence on every
+side; the vector fill by 3rd-order convergence to the analytic wind on every ghost edge.  The
+composition of the corners (pass 2 of the scalar fill) and of the vector fill is fixed by
+those only up to truncation order: their bitwise agreement with the reference is parity
+unpinned.  src/interpolation.py cannot be imported here to write vectors (it imports netCDF4,
+and cartopy through src/plot.py, neither of which this image has).
 """
 import numpy as np
 

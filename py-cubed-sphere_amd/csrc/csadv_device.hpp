@@ -227,6 +227,25 @@ __device__ __forceinline__ void st_at(T *base, uint32_t idx, T v) {
   *(T CSADV_GLOBAL *)(b + idx * (uint32_t)sizeof(T)) = v;
 }
 
+// Stores that every lane issues, with no branch around them: a lane with nothing to store passes
+// an offset past the buffer's end, and the buffer's range check drops its write.  Used where a
+// branch would make the count of outstanding memory operations uncertain: loads and stores share
+// one counter, and a wait for a load past an optional store has to assume the store was not
+// issued, waiting for that much more than it needs.  The buffer covers `bytes` bytes from `base`
+// (< 2^31); kNoStore is past the end of any such buffer.
+constexpr uint32_t kNoStore = 0x80000000u;
+constexpr int kBufferDword3 = 0x00020000;   // raw (untyped) 32-bit elements, no swizzle
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(void *base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, kBufferDword3);
+}
+typedef unsigned int csadv_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(csadv_u32x2, v), r, byte_off, 0, 0);
+}
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t r, uint32_t byte_off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, byte_off, 0, 0);
+}
+
 // The wind of one edge from a one-field basis W and its coefficient s at the step's time
 // (src/averaged_velocity.py:14-62 with RK1: the wind at t_n), in FP64, in one expression shared by
 // the wind kernel and the step kernel so that both give the same bits (s = 1 gives w, with a

@@ -4,6 +4,8 @@
 // src/flux.py, src/reconstruction_1d.py).  Included by the step_<dtype>_r<recon>.hip
 // translation units, one per precision and reconstruction, which instantiate launch_step_r.
 #pragma once
+#include <utility>
+
 #include "csadv_internal.hpp"
 
 // Work item = (solve, panel, tracer, strip, segment): one wavefront marches down strip_w interior
@@ -13,40 +15,56 @@
 //     Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by DPP lane shifts), update of
 //     row y-4 and its mass-fixer partial sum;
 //   * row stage for row y: x-flux of Q, split half-step Qx.
-// The windows are rings indexed by the iteration's phase modulo 6 (the loop is unrolled by
-// six, so every slot is a fixed register and nothing is shifted); row quantities that the
-// column stage needs four iterations later (metric, 1/g, wind) ride in the same rings.
-// The inputs of row y+PF are requested at the end of iteration y, PF = 2 rows ahead, so their
-// memory latency overlaps a whole iteration of arithmetic: the row's state, correction metric
-// and y-edge wind into one of two rotating buffers, the step-invariant tables and winds
-// straight into their ring slots.
+// The per-row values, loaded or computed, live in rings indexed by the row's offset from the
+// item's first row modulo the ring depth (the loop is unrolled by RD, a multiple of every
+// depth, so every slot is a fixed register and nothing is shifted).  The inputs of row y+PF
+// are requested at the end of iteration y, so their memory latency overlaps PF-1 whole
+// iterations of arithmetic.
 // (src/discrete_operators.py:18-101, src/flux.py, src/reconstruction_1d.py,
 // src/advection_timestep.py:43)
-template <typename T>
-struct StepIn {     // inputs of iteration y at the lane's column: row y and y-edge y-3
-  T q;                       // row y: Q
-  T gf;                      // row y: metric the correction total multiplies (gfix)
-  T raw;                     // row y: the stored value before the correction total is added
-  T qa;                      // first rows only: Q of row y in the other state buffer
-  T v, vs;                   // y-edge y-3: wind, upwind-side wind
-};
+
+// Rows whose inputs are in flight ahead of the row being processed.
+template <typename T> constexpr int kStepPrefetch = 2;
 
 // SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07); the metric treatment follows from it, since
 // the valid pairs are (AVLT or L04, MT-0) and (PL07, MT-PL07) (src/discrete_operators.py:48-82).
 template <typename T, int RECON, bool RK2, int SPLIT, bool WIND = false, bool XPULL = false>
 struct March {
   static constexpr int split = SPLIT;
-  // rows whose inputs are in flight ahead of the one being processed (a third row of state
-  // measured slower: its extra row offsets cost more than the longer lead saves)
-  static constexpr int PF = 2;
+  static constexpr int PF = kStepPrefetch<T>;
+  // Two arrangements of the per-row values, by precision (measured):
+  //  * EXACT (FP64, where a single solve leaves about two wavefronts per SIMD, so each
+  //    wavefront's waits are exposed): every per-row value, loaded or computed, sits in a ring
+  //    with one slot more than its lifetime needs, and the output row is stored after the next
+  //    row's loads by a store every lane issues.  The spare slot keeps a new value's register
+  //    apart from the one it replaces, whatever order the compiler gives the two stages of an
+  //    iteration; without it the register allocator rotated values at the loop's back-edge, and
+  //    a rotated value still in flight made the wavefront wait there for every outstanding
+  //    load.  Loads and stores share one counter, so a store ahead of the next row's loads, or
+  //    one behind a branch (whose issue the wait has to doubt), holds each row's wait back.
+  //    Together: 6-8% off the N=384 step kernel.
+  //  * otherwise (FP32, whose launches hold many wavefronts per SIMD and are bound by issue):
+  //    the loaded state of a row rotates through PF slots, and the output is stored where it
+  //    is formed.  The EXACT arrangement holds more registers here (126 against 96 in the
+  //    batched launch's kernel), which costs a wavefront per SIMD and 5% of that launch.
+  static constexpr bool EXACT = !std::is_same<T, float>::value;
+  // Ring depth: a row's loaded inputs live from the end of iteration r-PF to iteration r+4, its
+  // computed values from iteration r to r+5 (EXACT: one slot more).  RS: depth of the rings of
+  // the loaded state (Q, gfix, y-winds), which live from the end of iteration r-PF to r.
+  static constexpr int RD = PF + 4 + (EXACT ? 1 : 0);
+  static constexpr int RS = EXACT ? RD : PF;
+  static_assert(RD % RS == 0, "the state slot of a row follows from its ring phase");
   static constexpr bool mt0 = SPLIT != 3;
+  // ring slot of row y + j in phase K (the phase of row y is (y - y0) mod RD)
+  static constexpr int sl(int K, int j) { return ((K + j) % RD + RD) % RD; }
   // ---- inputs (per wavefront and tracer)
   const T *Q, *Qa, *U, *US, *V, *VS, *gc, *gu, *gv, *rgc, *gfix;
   // WIND: the winds of the step are formed here from the one-field FP64 basis (WU, WV) and its
   // coefficient ws at the step's time, as k_wind_avg forms them for RK1, instead of read
   const double *WU, *WV;
   double ws;
-  T *Qo;
+  T *Qo;                           // the output panel of the tracer ([M][ld])
+  __amdgpu_buffer_rsrc_t qo_buf;   // the same, for the stores of the EXACT arrangement
   int ld, ldu, M, iend, x, xc, xe, tj, ylast;
   int am2, am1, ap1, ap2;    // byte addresses of the lanes -2, -1, +1, +2 (mod 64) for lane pulls
   bool out_lane, need_col;
@@ -59,22 +77,26 @@ struct March {
   T *fside;
   bool w_lane, e_lane;
   int N;
-  // ---- state: rings of rows y-6 .. y-1 (slot of row y-6+i in iteration phase K: (K+i) % 6).
-  // The step-invariant row inputs (metric, 1/g, winds) are loaded straight into the slot their
-  // row will hold: with PF = 2 the loads of row y+2 are issued at the end of iteration y, when
-  // that slot's previous row, y-4, has had its last use.  gvr holds gv at y-edges by the same
-  // phase (edge y-2 in slot K % 6), whose previous edge, y-6, was last read at iteration y-2.
-  T qw[6], xw[6], gcw[6], rgw[6], uw[6], usw[6], guw[6], gvr[6];
-  T sw[6];                   // FP32: stored values S of the rows (the update is S + flux)
+  // ---- state: rings of rows (slot of row y+j in phase K: sl(K, j); in the state rings,
+  // sl(K, j) % RS).  The inputs of row y+PF are loaded at the end of iteration y straight into
+  // the slot their row will hold, whose previous row has had its last use by then.
+  //   loaded state, row r: qr Q as stored (S: the correction total is not in it), gfr the metric
+  //                   the correction total multiplies (gfix), qar Q of the other state buffer
+  //                   (first rows only); vr / vsr y-wind and its upwind-side wind at y-edge r-3
+  //   loaded tables, row r: gcw metric, rgw 1/g, uw / usw x-wind and its upwind-side wind,
+  //                   guw x-edge metric, gvr gv at y-edge r-2
+  //   computed, row r: qw Q (corrected), xw the split half-step Qx, sw (FP32) Q as stored
+  T qr[RS], gfr[RS], qar[RS], vr[RS], vsr[RS];
+  T qw[RD], xw[RD], gcw[RD], rgw[RD], uw[RD], usw[RD], guw[RD], gvr[RD];
+  T sw[RD];
   T g1prev, g2prev, vprev;
   T pa_A, pa_B, pa_q6, pb_A, pb_B, pb_q6;   // row y-4 as the left (upwind) cell of edge y-3: Q, Qx
   double part;
 
-  // Inputs of row y, whose ring slot is SLOT.  SPEC: also read Q from the other state buffer
-  // (Qa), for the rows requested before the step counter has arrived.
+  // Inputs of row y into ring slot SLOT.  SPEC: also read Q from the other state buffer (Qa),
+  // for the rows requested before the step counter has arrived.
   template <bool SPEC, int SLOT>
-  __device__ __forceinline__ void load(int y, StepIn<T> &r) {
-    static_assert(PF == 2, "the ring slots are reused two rows ahead");
+  __device__ __forceinline__ void load(int y) {
     const int yc = min(y, M - 1);
     const uint32_t o = (uint32_t)(yc * ld + xc), ou = (uint32_t)(yc * ldu + xe);
     if constexpr (WIND) uw[SLOT] = T(wind_of_basis(ws, ld_at(WU, ou)));
@@ -83,35 +105,37 @@ struct March {
     guw[SLOT] = ld_at(gu, ou);
     gcw[SLOT] = ld_at(gc, o);
     rgw[SLOT] = ld_at(rgc, o);
-    r.gf = ld_at(gfix, o);
-    const bool reach = need_col && y >= 1 && y < iend + 3;
-    if constexpr (std::is_same<T, float>::value) {
-      // FP32: Q of every lane is read (clamped row and column: rows past ylast are requested
-      // too) and zeroed outside the stencils' reach by a select, with no branch around the load
-      // (measured faster for FP32, slower for FP64, whose registers it raises)
+    constexpr int SS = SLOT % RS;          // state slot
+    gfr[SS] = ld_at(gfix, o);
+    // Q of every lane, at the clamped row and column (rows past ylast are requested too): no
+    // branch around the load, and the value outside the stencils' reach is replaced by zero
+    // where the row is used (see reach), not here, where the select would wait for the data
+    if constexpr (EXACT) {
+      qr[SS] = ld_at(Q, o);
+      if constexpr (SPEC) qar[SS] = ld_at(Qa, o);
+    } else {   // zeroed outside the stencils' reach here (fewer registers than where it is used)
+      const bool rch = reach(y);
       const T qv = ld_at(Q, o);
-      r.q = reach ? qv : T(0);
+      qr[SS] = rch ? qv : T(0);
       if constexpr (SPEC) {
         const T qa = ld_at(Qa, o);
-        r.qa = reach ? qa : T(0);
-      }
-    } else {
-      r.q = T(0);    // rows outside the stencils' reach
-      r.qa = T(0);
-      if (reach) {
-        r.q = ld_at(Q, (uint32_t)(y * ld + x));
-        if constexpr (SPEC) r.qa = ld_at(Qa, (uint32_t)(y * ld + x));
+        qar[SS] = rch ? qa : T(0);
       }
     }
     const int ec = min(max(y - 3, 0), M - 1);
-    if constexpr (WIND) r.v = T(wind_of_basis(ws, ld_at(WV, (uint32_t)(ec * ld + xc))));
-    else r.v = ld_at(V, (uint32_t)(ec * ld + xc));
-    r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
+    if constexpr (WIND) vr[SS] = T(wind_of_basis(ws, ld_at(WV, (uint32_t)(ec * ld + xc))));
+    else vr[SS] = ld_at(V, (uint32_t)(ec * ld + xc));
+    if constexpr (RK2) vsr[SS] = ld_at(VS, (uint32_t)(ec * ld + xc));
     gvr[SLOT] = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
   }
 
-  // the upwind-side x-wind of the row in a slot (RK1: the wind itself)
+  // whether row y at this lane's column is within the stencils' reach (the columns and rows of
+  // the panel's 3-cell halo and interior; the array's outer ghost layer is never read)
+  __device__ __forceinline__ bool reach(int y) const { return need_col && y >= 1 && y < iend + 3; }
+
+  // the upwind-side winds of the row in a slot (RK1: the winds themselves)
   __device__ __forceinline__ T wind_sign(int slot) const { return RK2 ? usw[slot] : uw[slot]; }
+  __device__ __forceinline__ T vwind_sign(int slot) const { return RK2 ? vsr[slot] : vr[slot]; }
 
   // every cell the stencils read carries the mass-fixer correction total C: Q = S + C gfix,
   // gfix being g inside the panel and, at a ghost cell, the weighted metric of the cells its
@@ -119,10 +143,9 @@ struct March {
   // its FP32 rounding is far below that of the sum, which is rounded once).  Lanes and rows
   // outside the stencils' reach hold 0 + C gfix, which no stored cell reads.
   // (with no correction total, cf = 0 and q + 0 gfix = q: no branch on the mass fixer)
-  __device__ __forceinline__ void correct(StepIn<T> &r) const {
-    r.raw = r.q;
-    if constexpr (std::is_same<T, float>::value) r.q = (r.q + off) + T(cf) * r.gf;
-    else r.q = r.q + T(cf) * r.gf;
+  __device__ __forceinline__ T corrected(T raw, T gf) const {
+    if constexpr (std::is_same<T, float>::value) return (raw + off) + T(cf) * gf;
+    else return raw + T(cf) * gf;
   }
 
   // Iteration y in phase K.  STAGE: how much of the column stage has begun, fixed by the row's
@@ -131,19 +154,22 @@ struct March {
   // tj+4 on (3).  The first seven rows are peeled with their stage; every later row is 3, so
   // the loop carries no per-row test.
   template <int K, int STAGE>
-  __device__ __forceinline__ void body(const int y, StepIn<T> &in) {
-    constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
-                  s5 = (K + 5) % 6;
-    correct(in);
+  __device__ __forceinline__ void body(const int y) {
+    // slots of rows y, y-5 .. y-1, and of row y in the state rings
+    constexpr int s0 = sl(K, 0), s1 = sl(K, -5), s2 = sl(K, -4), s3 = sl(K, -3), s4 = sl(K, -2), s5 = sl(K, -1);
+    constexpr int ss = s0 % RS;
+    const T raw = !EXACT || reach(y) ? qr[ss] : T(0);
+    const T q = corrected(raw, gfr[ss]);
     const T gvm1 = gvr[s4], gv0 = gvr[s5], gvp1 = gvr[s0];   // gv at y-edges y-4, y-3, y-2
+    T out = T(0);
     // ---- column stage: parabolas of row y-3, y-edge e = y-3, output row r = y-4
     if constexpr (STAGE >= 1) {
       const T gce = gcw[s3];
       const Parab<T> a = parabola<T, RECON>(qw[s1], qw[s2], qw[s3], qw[s4], qw[s5], gv0, gvp1, gce, mt0);
       const Parab<T> b = parabola<T, RECON>(xw[s1], xw[s2], xw[s3], xw[s4], xw[s5], gv0, gvp1, gce, mt0);
       if constexpr (STAGE >= 2) {
-        const T v = in.v;
-        const bool vup = in.vs >= T(0);
+        const T v = vr[ss];
+        const bool vup = vwind_sign(ss) >= T(0);
         const T cy = v * dtdy;
         const T g1 = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, gv0, mt0);     // times dt/dy
         const T g2 = upwind<T>(vup, pb_A, pb_B, pb_q6, b, cy, gv0, mt0);
@@ -168,13 +194,13 @@ struct March {
           const T dG2 = g2prev - g2;
           // div = -(dF2 + dG2) / (dt g);  Q - dt div = Q + (dF2 + dG2) / g
           const T flx = dF2 + dG2;
-          if (out_lane) {
-            // the correction total stays out of the stored state: S(n+1) = S(n) + flux
-            T out;
-            if constexpr (std::is_same<T, float>::value) out = sw[s2] + flx * rg;
-            else out = (qw[s2] + flx * rg) - T(cf) * gcm;
-            st_at(Qo, (uint32_t)((y - 4) * ld + x), out);
-            part -= (double)flx;               // used with MF-PR only
+          // the correction total stays out of the stored state: S(n+1) = S(n) + flux
+          if constexpr (EXACT) {
+            out = (qw[s2] + flx * rg) - T(cf) * gcm;
+            if (out_lane) part -= (double)flx;   // used with MF-PR only
+          } else if (out_lane) {
+            st_at(Qo, (uint32_t)((y - 4) * ld + x), sw[s2] + flx * rg);
+            part -= (double)flx;
           }
           if (fside) {   // MF-AF: cube-edge fluxes of output row r = y - 4 (f2: left edge of the lane's cell)
             const int r = y - 4;
@@ -192,7 +218,7 @@ struct March {
       pb_A = b.rA; pb_B = b.rB; pb_q6 = b.q6;
     }
     // ---- row stage: row y
-    const T q = in.q, u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
+    const T u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
     const bool up = wind_sign(s0) >= T(0);
     const T gup1 = from_right(gu0);
     T q_m2, q_m1, q_p1, q_p2;
@@ -206,48 +232,64 @@ struct March {
       c2 = gu0 * (u * dtdx);
     }
     const T qx = split_update<T>(split, q, dF1, gc0, rgw[s0], c1, c2);
-    // row y replaces row y-6 in the rings of computed values (the loaded ones are there already)
     qw[s0] = q; xw[s0] = qx;
-    if constexpr (std::is_same<T, float>::value) sw[s0] = in.raw;
+    if constexpr (std::is_same<T, float>::value) sw[s0] = raw;
     // rows past ylast are read in bounds (clamped rows, masked Q) and never used
-    load<false, (K + PF) % 6>(y + PF, in);
+    load<false, sl(K, PF)>(y + PF);
+    // EXACT: the output row's store, after the loads and issued by every lane (the first rows,
+    // which have no output row yet, issue it too, dropped)
+    if constexpr (EXACT)
+      st_buf(qo_buf, STAGE >= 3 && out_lane ? (uint32_t)((y - 4) * ld + x) * (uint32_t)sizeof(T) : kNoStore, out);
+  }
+
+  template <int... J>
+  __device__ __forceinline__ void first_rows(std::integer_sequence<int, J...>) {
+    (load<true, J>(tj - 3 + J), ...);
   }
 
   // The first PF rows' inputs, requested before the step counter is known: Q from both state
   // buffers (Q = q0, Qa = q1), the rest from the step-invariant tables and winds.
-  __device__ __forceinline__ void start(StepIn<T> *b) {
+  __device__ __forceinline__ void start() {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = gvr[k] = sw[k] = T(0);
+    for (int k = 0; k < RD; ++k) qw[k] = xw[k] = gcw[k] = rgw[k] = uw[k] = usw[k] = guw[k] = gvr[k] = sw[k] = T(0);
+#pragma unroll
+    for (int k = 0; k < RS; ++k) qr[k] = gfr[k] = qar[k] = vr[k] = vsr[k] = T(0);
     g1prev = g2prev = vprev = T(0);   // gv is complete from y = tj-1 on; the column stage starts at tj+2
     pa_A = pa_B = pa_q6 = pb_A = pb_B = pb_q6 = T(0);
     part = 0.0;
     // y0 = tj - 3 has phase 0; ylast - y0 = ny + 6 >= PF: every initial row exists
-    load<true, 0>(tj - 3, b[0]);
-    load<true, 1>(tj - 2, b[1]);
+    first_rows(std::make_integer_sequence<int, PF>());
   }
 
   // The step's input buffer is known: keep the first rows' Q from it and read the rest there.
-  __device__ __forceinline__ void resolve(StepIn<T> *b, bool odd) {
+  __device__ __forceinline__ void resolve(bool odd) {
 #pragma unroll
-    for (int k = 0; k < PF; ++k) b[k].q = odd ? b[k].qa : b[k].q;
+    for (int k = 0; k < PF; ++k) qr[k % RS] = odd ? qar[k % RS] : qr[k % RS];
     Q = odd ? Qa : Q;
   }
 
-  __device__ __forceinline__ void run(StepIn<T> *b) {
+  // row y0 + K0 + j in phase (K0 + j) % RD
+  template <int K0, int STAGE, int J>
+  __device__ __forceinline__ void row(int y) {
+    body<(K0 + J) % RD, STAGE>(y + J);
+  }
+  template <int K0, int... J>
+  __device__ __forceinline__ void full_rows(int y, std::integer_sequence<int, J...>) {
+    (row<K0, 3, J>(y), ...);
+  }
+  template <int K0, int... J>
+  __device__ __forceinline__ void last_rows(int y, std::integer_sequence<int, J...>) {
+    ((y + J <= ylast ? row<K0, 3, J>(y) : void()), ...);
+  }
+
+  __device__ __forceinline__ void run() {
     // rows y0 .. y0+6 (ylast - y0 = ny + 6 >= 7: they all exist), the column stage starting
     int y = tj - 3;
-    body<0, 0>(y, b[0]); body<1, 0>(y + 1, b[1]); body<2, 0>(y + 2, b[0]); body<3, 0>(y + 3, b[1]);
-    body<4, 0>(y + 4, b[0]); body<5, 1>(y + 5, b[1]); body<0, 2>(y + 6, b[0]);
-    // rows y0+7 on: the full step, six rows (phases 1 .. 5, 0) per iteration
-    for (y += 7; y + 5 <= ylast; y += 6) {
-      body<1, 3>(y, b[1]); body<2, 3>(y + 1, b[0]); body<3, 3>(y + 2, b[1]);
-      body<4, 3>(y + 3, b[0]); body<5, 3>(y + 4, b[1]); body<0, 3>(y + 5, b[0]);
-    }
-    if (y <= ylast) body<1, 3>(y, b[1]);
-    if (y + 1 <= ylast) body<2, 3>(y + 1, b[0]);
-    if (y + 2 <= ylast) body<3, 3>(y + 2, b[1]);
-    if (y + 3 <= ylast) body<4, 3>(y + 3, b[0]);
-    if (y + 4 <= ylast) body<5, 3>(y + 4, b[1]);
+    row<0, 0, 0>(y); row<0, 0, 1>(y); row<0, 0, 2>(y); row<0, 0, 3>(y);
+    row<0, 0, 4>(y); row<0, 1, 5>(y); row<0, 2, 6>(y);
+    // rows y0+7 on: the full step, RD rows per iteration
+    for (y += 7; y + RD - 1 <= ylast; y += RD) full_rows<7>(y, std::make_integer_sequence<int, RD>());
+    last_rows<7>(y, std::make_integer_sequence<int, RD - 1>());
   }
 };
 
@@ -297,7 +339,6 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   const int64_t off = tr * S.tstride + p * S.pstride;
   m.Q = reinterpret_cast<const T *>(D.q0) + off;
   m.Qa = reinterpret_cast<const T *>(D.q1) + off;
-  StepIn<T> b[March<T, RECON, RK2, SPLIT, WIND, XPULL>::PF];
   int step;
   if constexpr (WIND) {
     // the winds of the first rows need the step's coefficient: the step counter comes first
@@ -306,9 +347,9 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
     m.ws = D.coef[row * 2];
     m.WU = D.wbu + p * S.pstride_u;
     m.WV = D.wbv + p * S.pstride_v;
-    m.start(b);
+    m.start();
   } else {
-    m.start(b);
+    m.start();
     // read after the first rows are requested: a wait for any scalar load waits for all of them
     // (they return out of order), so reading these first would hold back the rows' loads
     step = D.ctl[2];
@@ -317,10 +358,12 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
   m.off = (std::is_same<T, float>::value && D.offset) ? T(D.offset[tr]) : T(0);
   if (step < 0 || (D.stop > 0 && step >= D.stop)) return;   // solve halted or finished
   const bool odd = step & 1;                  // the state of step n lives in q[n & 1]
-  m.resolve(b, odd);
+  m.resolve(odd);
   m.Qo = reinterpret_cast<T *>(odd ? D.q0 : D.q1) + off;
+  if constexpr (March<T, RECON, RK2, SPLIT, WIND, XPULL>::EXACT)
+    m.qo_buf = buffer_of(reinterpret_cast<T *>(odd ? D.q0 : D.q1) + off, (uint32_t)((int64_t)m.M * m.ld * sizeof(T)));
   m.cf = odd ? c1 : c0;
-  m.run(b);
+  m.run();
   if (mfpr) {
     const double part = warp_sum(m.part) / D.dt;   // sum of div g over the wavefront's cells
     if (lane == 0) D.partials[(int64_t)tr * S.part_per_tracer + item] = part;
