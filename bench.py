@@ -82,9 +82,10 @@ OPTION_NAMES = {
 FORMS = ("nl10_n768_f32", "batched_64_f32", "served_poisson_f32")
 SERVED_REQUESTS, SERVED_RATE = 200, 12.0      # Poisson stream: count, arrivals per second
 BATCH_NS = (32, 48, 96, 192, 384, 512)
-# tile ids from the autotuner for the single-solve workloads (the batched workload tunes a tile
-# per N at start-up over every tile shape)
-TUNED_TILE = {("f64", 384): 2, ("f32", 768): 3, ("f64", 48): 1}
+# tile ids from sweeps of every tile shape for the single-solve workloads (the batched workload
+# tunes a tile per N at start-up over every tile shape)
+TUNED_TILE = {"two_hills_n384_f64": 2, "nl10_n768_f32": 3, "polar_n48_f64": 1,
+              "two_hills_n384_f64_etpl07": 0, "two_hills_n384_f64_rk2_mfaf": 2}
 POOL_MAX = 16           # the host cores one GPU's share of the box holds
 
 
@@ -296,7 +297,7 @@ def run_workload(name, K=None, W=10, graph=16, tile=None, seed=2024, cpu_seconds
     tuned = None
     if kind == "single":
         specs = [(N, workload(case_name), nstep_total)]
-        tile = tile if tile is not None else TUNED_TILE.get((dtype, N), 0)
+        tile = tile if tile is not None else TUNED_TILE.get(name, 0)
     else:
         specs = batch_specs(seed, nstep_total)
         if tile is None:

@@ -101,8 +101,10 @@ typedef struct csadv_solve_desc {
                                       number of steps while its solves end at their own step);
                                       0 = no limit                                           */
   void *work;                      /* general step (ET-S72, ET-PL07): scratch of csadv_work_bytes()
-                                      bytes; fused step with MF-AF: ntr * 24 * N elements of the
-                                      dtype (the second pass's cube-edge fluxes); else unused */
+                                      bytes (the general step takes N up to 6680 in FP64:
+                                      CSADV_E_RANGE past that); fused step with
+                                      MF-AF: ntr * 24 * N elements of the dtype (the second
+                                      pass's cube-edge fluxes); else unused */
   double *offset;                  /* fused step, FP32: [ntr] constants (exact in FP32) the stored
                                       state is kept from, Q = S + C gfix + offset, so that
                                       increments far below an ulp of Q (on a flat background)

@@ -32,6 +32,11 @@ int validate(const csadv_solve_desc &d, std::string &msg) {
     msg = "the general step expects ld = round_up(N+8, 4) and ldu = round_up(N+9, 4) (the csadv_work_bytes layout)";
     return CSADV_E_ARG;
   }
+  if (is_general(d) && general_pair_bytes(d.N, d.dtype) >= (int64_t(1) << 32)) {
+    msg = "the general step (ET-S72, ET-PL07) addresses a tracer's Qx and Qy with 32-bit byte offsets: "
+          "N up to 6680 in FP64";
+    return CSADV_E_RANGE;
+  }
   if (d.nbasis < 1 || d.nbasis > 8) { msg = "nbasis must be 1..8"; return CSADV_E_RANGE; }
   if (d.unsteady && d.max_steps < 1) { msg = "unsteady winds need a coefficient table"; return CSADV_E_ARG; }
   if (!d.q0 || !d.q1 || !d.uavg || !d.vavg || !d.gc || !d.gu || !d.gv || !d.rgc || !d.gc64 || !d.lagw ||

@@ -102,6 +102,11 @@ inline int general_ldu(int N) { return (N + 2 * NG + 4) / 4 * 4; }
 // The options of one solve that need the general step: ET-S72 and ET-PL07 (ET-DG runs the
 // fused step, with MF-AF through its cube-edge fix-up).
 inline bool is_general(const csadv_solve_desc &d) { return d.et != 3; }
+// Bytes of one tracer's Qx and Qy in the general step's work buffer (two cell arrays [6][M][ld]):
+// the passes address them from one base with 32-bit byte offsets.
+inline int64_t general_pair_bytes(int N, int dtype) {
+  return int64_t(2) * 6 * (N + 2 * NG) * general_ld(N) * (dtype == CSADV_F64 ? 8 : 4);
+}
 
 // Elements (of the state's dtype) of the work buffer the fused step needs: MF-AF keeps the
 // second pass's cube-edge fluxes, [ntr][6][4][N]; none otherwise.

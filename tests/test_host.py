@@ -241,6 +241,28 @@ def test_create_rejects_invalid_descriptor_without_gpu_work():
     lib.csadv_destroy(h)
 
 
+def test_create_rejects_a_general_step_past_its_addressing_limit():
+    """The general step addresses a tracer's Qx and Qy (two [6][M][ld] arrays) from one base
+    with 32-bit byte offsets: in FP64 N = 6680 passes that check and N = 6681 is rejected with
+    CSADV_E_RANGE before any GPU work; FP32 fits up to the library's N = 8192."""
+    from cubedsphere import _lib
+    lib = _lib.load()
+    for dtype, N, want in ((0, 6680, None), (0, 6681, -4), (1, 8192, None)):
+        d = _lib.SolveDesc()
+        d.N, d.ntr, d.dtype = N, 1, dtype
+        d.ld, d.ldu = (N + 11) // 4 * 4, (N + 12) // 4 * 4
+        d.recon, d.dp, d.split, d.et, d.mt, d.mf, d.nbasis = 3, 1, 3, 2, 2, 1, 1
+        d.work = 1                                  # (never dereferenced: validation only)
+        h = ctypes.c_void_p()
+        rc = lib.csadv_create(ctypes.byref(d), 1, 0, ctypes.byref(h))
+        msg = lib.csadv_last_error(h)
+        lib.csadv_destroy(h)
+        if want is None:
+            assert b"32-bit" not in msg, msg        # fails later, on the null device pointers
+        else:
+            assert rc == want and b"32-bit" in msg, (rc, msg)
+
+
 # ------------------------------------------------------------------ cube-edge pairings
 def _side_index(side, N, i0):
     """(array axis, boundary index, ghost index) of a side in reference layout [i, j, p]."""
