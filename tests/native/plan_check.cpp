@@ -59,7 +59,7 @@ int main() {
   int cases = 0;
   std::string msg;
   // ---- validation: a valid descriptor passes; each broken field is rejected with a message
-  for (int N : {8, 9, 48, 384, 1024, 8192}) {
+  for (int N : {8, 9, 48, 384, 1024, 6680, 6681, 8192}) {
     csadv_solve_desc d = valid(N, 4, arena);
     CHECK(validate(d, msg) == CSADV_OK, "valid N=%d rejected: %s", N, msg.c_str());
     ++cases;
@@ -98,10 +98,13 @@ int main() {
       CHECK(rc < 0 && !msg.empty(), "broken descriptor %zu of N=%d accepted", k, N);
       ++cases;
     }
-    // the general step with its work buffer and strides passes
+    // the general step with its work buffer and strides passes up to its addressing limit
+    // (a tracer's Qx and Qy within 4 GiB of their base) and is rejected past it
     csadv_solve_desc g = d;
     g.et = 2; g.split = 3; g.mt = 2; g.mf = 1; g.work = arena.data();
-    CHECK(validate(g, msg) == CSADV_OK, "general N=%d rejected: %s", N, msg.c_str());
+    const bool fits = general_pair_bytes(N, g.dtype) < (int64_t(1) << 32);
+    const int grc = validate(g, msg);
+    CHECK(fits ? grc == CSADV_OK : grc == CSADV_E_RANGE, "general N=%d dtype %d: %d %s", N, g.dtype, grc, msg.c_str());
     ++cases;
   }
   // ---- handle compatibility
