@@ -17,7 +17,7 @@
 //   k_wind_avg                        time-averaged winds (src/averaged_velocity.py:14-62)
 //   k_settle / k_finalize / k_clear_corr, k_diag / k_diag_final   state read-out and diagnostics
 // The fused step kernel lives in csadv_step.hpp (instantiated by step_<dtype>_r<recon>.hip), the
-// general step in csadv_general.hpp (general_<dtype>.hip), the host-side planning in
+// general step in csadv_general.hpp (general_<dtype>_s<split>.hip), the host-side planning in
 // csadv_plan.cpp.
 #include "csadv_internal.hpp"
 

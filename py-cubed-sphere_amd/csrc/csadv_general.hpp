@@ -28,9 +28,12 @@
 // each direction (interior rows / columns only), as the reference's arrays hold them when its
 // flux reads them.
 //
-// Included by general_f64.hip and general_f32.hip, which instantiate enqueue_general_t.
+// Included by the general_<dtype>_s<split>.hip translation units, one per precision and
+// splitting, which instantiate enqueue_general_split.
 
 #pragma once
+#include <utility>
+
 #include "csadv_internal.hpp"
 
 namespace gp {
@@ -100,78 +103,125 @@ template <typename T> __device__ __forceinline__ T pl_right(T m2, T m1, T c0, T 
   return T(2.0 / 60.0) * m2 + T(-13.0 / 60.0) * m1 + T(47.0 / 60.0) * c0 + T(27.0 / 60.0) * p1 + T(-3.0 / 60.0) * p2;
 }
 
-template <typename T>
-struct GIn {        // inputs of iteration y at the lane's column
-  T f;              // row y of the field the row stage reconstructs (pass 1: Q; pass 2: Qy)
-  T g;              // pass 2: row y of the field the column stage reconstructs (Qx)
-  T v, vs;          // y-edge y-3: wind, upwind-side wind
-};
-
 // One wavefront of a pass: strip columns ti .. ti+nx-1 (lane l <-> column ti-3+l), rows
 // tj .. tj+ny-1.  Iteration y (rows y0 = tj-3 .. ylast = tj+ny+3):
-//   * column stage at y-edge y-3 from the six-row ring of the y-field (rows y-6 .. y-1): the
+//   * column stage at y-edge y-3 from the ring of the y-field (rows y-6 .. y-1): the
 //     parabola of row y-3, the y-flux through edge y-3, then for row y-4 the y-split Qy
 //     (pass 1) or the update (pass 2, with the x-flux difference the row stage kept);
 //   * row stage for row y: x-window of the x-field by DPP, parabolas, x-fluxes (every lane the
 //     flux through its cell's left edge), then Qx (pass 1) or the x-flux difference (pass 2).
-// The inputs of row y+PF are requested at the end of iteration y, PF = 2 rows ahead.
-template <typename T, int RECON, bool RK2, int PASS>
+// The inputs of row y+PF are requested at the end of iteration y, PF = 2 rows ahead.  Loads and
+// stores share one counter, and a wait for a load also waits for everything issued before it,
+// so the march keeps that count exact: every per-row value sits in a ring one slot deeper than
+// its lifetime (RD), which keeps the register allocator from rotating values still in flight
+// at the loop's back-edge; every lane issues every load (a lane with no source in a row regime
+// reads element 0 and masks the value where the row is used); the row's stores follow the next
+// row's loads and every lane issues them (a lane with nothing to store passes an offset past
+// the buffer, which drops it).  The field offsets advance by one row stride per row and are
+// recomputed only where the row regime changes.
+// SPLIT is the splitting (1 AVLT, 2 L04, 3 PL07), with the metric treatment it implies (MT-0
+// with AVLT and L04, MT-PL07 with PL07); a compile-time constant, as in the fused step.
+template <typename T, int RECON, bool RK2, int PASS, int SPLIT>
 struct GMarch {
   static constexpr int PF = 2;
+  static constexpr int split = SPLIT;
+  static constexpr bool mt0 = SPLIT != 3;
+  static constexpr int RD = PF + 5;
+  // ring slot of row y + j in phase K (the phase of row y is (y - y0) mod RD)
+  static constexpr int sl(int K, int j) { return ((K + j) % RD + RD) % RD; }
   // ---- per wavefront
   const T *X;              // pass 1: Q of the tracer; pass 2: Qx of the tracer (Qy = Qx + cell)
   const T *Qs;             // pass 2: Q of the tracer (the update's input)
-  T *Out;                  // pass 1: Qx of the tracer (Qy = Qx + cell); pass 2: the new Q
-  T *Div;                  // pass 2 with MF-PR: div of the tracer (the update follows later)
+  // pass 1: Qx and Qy of the tracer ([2][cell]); pass 2: the new Q, or with MF-PR its div
+  __amdgpu_buffer_rsrc_t out;
+  bool Div;                // pass 2: out holds div (MF-PR; the update follows later)
   T *fside;                // pass 2 with MF-AF: cube-edge fluxes of the panel, [E, W, N, S][N]
   const T *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
   Lines lf, lg;            // lines of the row stage's field and (pass 2) the column stage's
-  int ld, ldu, N, i0, iend, x, xc, xe, tj, tend, ylast, cell, own, split;
-  bool mt0, pl07, sp_x, out_lane, w_lane, e_lane;
+  bool okf[3], okg[3];     // per row regime: the lane has a source of that field there
+  int ld, ldu, N, i0, iend, x, xc, xe, tj, tend, ylast, cell, own;
+  bool pl07, sp_x, out_lane, w_lane, e_lane;
   bool lo_g, lo_b, lo_1, hi_1, hi_b, hi_g;   // column x is i0-1, i0, i0+1, iend-2, iend-1, iend
   T dtdx, dtdy, dtc;
-  // ---- rings: six rows (slot of row y-6+i in phase K: (K+i) % 6), winds of three rows
-  T fw[6];                 // the column stage's field (pass 1: Q, pass 2: Qx), rows y-6 .. y-1
-  T dfw[6];                // pass 2: x-flux difference of the rows
-  T qsw[6];                // pass 2: Q of the rows (loaded straight into their slots)
-  T gcw[6], rgw[6], gvr[6];   // metric, 1/metric of the rows; gv at y-edges (edge y-2 in slot K)
-  T uw[3], usw[3], guw[3];    // x-edge wind, upwind-side wind, gu of row y in slot K % 3
+  // ---- the next row to load: its row and regime, the element offsets of the fields and of Q
+  // there (of, og, oq) and the fields' row strides in that regime (bf, bg)
+  int lrow, lreg;
+  int of, og, oq, bf, bg;
+  // ---- rings (slot of row y+j in phase K: sl(K, j))
+  T fr[RD], gr[RD];        // loaded, unmasked: the row stage's field, (pass 2) the column stage's
+  T vr[RD], vsr[RD];       // y-wind and its upwind-side wind at y-edge y-3 of the row's iteration
+  T fw[RD];                // the column stage's field (pass 1: Q, pass 2: Qx), masked
+  T dfw[RD];               // pass 2: x-flux difference of the rows
+  T qsw[RD];               // pass 2: Q of the rows
+  T gcw[RD], rgw[RD], gvr[RD];   // metric, 1/metric of the rows; gv at y-edge y-2 in row y's slot
+  T uw[RD], usw[RD], guw[RD];    // x-edge wind, upwind-side wind, gu of the rows
   T gprev, vprev, pa_A, pa_B, pa_q6;
   double part;             // pass 2 with MF-PR: sum of div g over the wavefront's cells
 
-  __device__ __forceinline__ T field(const Lines &L, int y, int reg) const {
+  // row regime of row y: 0 south ghost rows, 1 interior, 2 north ghost rows (wave-uniform)
+  __device__ __forceinline__ int regime(int y) const { return y < i0 ? 0 : (y < iend ? 1 : 2); }
+  // the line (a, b) of regime reg (a lane with no source there has a = b = 0: element 0)
+  __device__ __forceinline__ void line(const Lines &L, int reg, int &a, int &b) const {
     // (the members are read into values first: a select between member lvalues would keep the
     // march's state in scratch memory)
     const int a0 = L.a0, a1 = L.a1, a2 = L.a2, b0 = L.b0, b1 = L.b1, b2 = L.b2;
-    const int a = reg == 0 ? a0 : (reg == 1 ? a1 : a2);
-    const int b = reg == 0 ? b0 : (reg == 1 ? b1 : b2);
-    T v = T(0);
-    if ((L.ok >> reg) & 1u) v = ld_at(X, (uint32_t)(a + y * b));
-    return v;
+    a = reg == 0 ? a0 : (reg == 1 ? a1 : a2);
+    b = reg == 0 ? b0 : (reg == 1 ? b1 : b2);
+  }
+  __device__ __forceinline__ bool has(const bool *ok, int reg) const {
+    const bool k0 = ok[0], k1 = ok[1], k2 = ok[2];
+    return reg == 0 ? k0 : (reg == 1 ? k1 : k2);
+  }
+  // one value of a field at row y, zero where the lane has no source (the first parabola row of
+  // a segment next to the north side reads one row before the march)
+  __device__ __forceinline__ T field(const Lines &L, const bool *ok, int y) const {
+    int a, b;
+    const int reg = regime(y);
+    line(L, reg, a, b);
+    const T v = ld_at(X, (uint32_t)(a + y * b));
+    return has(ok, reg) ? v : T(0);
   }
 
-  // Inputs of row y into ring slots S6 (six-row rings) and S3 (wind rings).
-  template <int S6, int S3>
-  __device__ __forceinline__ void load(int y, GIn<T> &r) {
+  // Inputs of row y into ring slot S.
+  template <int S>
+  __device__ __forceinline__ void load(int y) {
     y = min(y, ylast);       // rows past the last are requested in bounds and never used
-    const int reg = y < i0 ? 0 : (y < iend ? 1 : 2);
+    if (y != lrow) {         // (wave-uniform) advance the offsets one row, or set them anew
+      const int reg = regime(y);
+      if (reg == lreg && y == lrow + 1) {
+        of += bf;
+        if constexpr (PASS == 2) og += bg;
+        oq += ld;
+      } else {
+        int a, b;
+        line(lf, reg, a, b);
+        of = a + y * b;
+        bf = b;
+        if constexpr (PASS == 2) {
+          line(lg, reg, a, b);
+          og = a + y * b;
+          bg = b;
+        }
+        oq = own + y * ld;
+      }
+      lrow = y;
+      lreg = reg;
+    }
     const uint32_t o = (uint32_t)(y * ld + xc), ou = (uint32_t)(y * ldu + xe);
-    uw[S3] = ld_at(U, ou);
-    if constexpr (RK2) usw[S3] = ld_at(US, ou);
-    guw[S3] = ld_at(gu, ou);
-    gcw[S6] = ld_at(gc, o);
-    rgw[S6] = ld_at(rgc, o);
-    r.f = field(lf, y, reg);
+    uw[S] = ld_at(U, ou);
+    if constexpr (RK2) usw[S] = ld_at(US, ou);
+    guw[S] = ld_at(gu, ou);
+    gcw[S] = ld_at(gc, o);
+    rgw[S] = ld_at(rgc, o);
+    fr[S] = ld_at(X, (uint32_t)of);
     if constexpr (PASS == 2) {
-      r.g = field(lg, y, reg);
-      T q = T(0);
-      if (out_lane && y >= tj && y < tend) q = ld_at(Qs, (uint32_t)(own + y * ld));
-      qsw[S6] = q;
+      gr[S] = ld_at(X, (uint32_t)og);
+      qsw[S] = ld_at(Qs, (uint32_t)oq);   // (at the clamped column: every lane reads in bounds)
     }
     const int ec = max(y - 3, 0);
-    r.v = ld_at(V, (uint32_t)(ec * ld + xc));
-    r.vs = RK2 ? ld_at(VS, (uint32_t)(ec * ld + xc)) : r.v;
-    gvr[S6] = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
+    vr[S] = ld_at(V, (uint32_t)(ec * ld + xc));
+    if constexpr (RK2) vsr[S] = ld_at(VS, (uint32_t)(ec * ld + xc));
+    gvr[S] = ld_at(gv, (uint32_t)((ec + 1) * ld + xc));
   }
 
   // ET-PL07 along x: the lane's edge values when its column is next to a panel side (the
@@ -208,28 +258,33 @@ struct GMarch {
   // Iteration y in phase K; STAGE as in the fused step: parabolas of row y-3 from y = tj+2 on
   // (>= 1), fluxes through y-edge y-3 from tj+3 on (>= 2), row y-4's result from tj+4 on (3).
   template <int K, int STAGE>
-  __device__ __forceinline__ void body(const int y, GIn<T> &in) {
-    constexpr int s0 = K % 6, s1 = (K + 1) % 6, s2 = (K + 2) % 6, s3 = (K + 3) % 6, s4 = (K + 4) % 6,
-                  s5 = (K + 5) % 6, w0 = K % 3;
-    const T cur = PASS == 1 ? in.f : in.g;                   // row y of the column stage's field
+  __device__ __forceinline__ void body(const int y) {
+    // slots of rows y, y-5 .. y-1 and y-6
+    constexpr int s0 = sl(K, 0), s1 = sl(K, -5), s2 = sl(K, -4), s3 = sl(K, -3), s4 = sl(K, -2), s5 = sl(K, -1),
+                  s6 = sl(K, -6);
+    const int reg = regime(y);
+    const T f = has(okf, reg) ? fr[s0] : T(0);               // row y of the row stage's field
+    T cur = f;                                                // row y of the column stage's field
+    if constexpr (PASS == 2) cur = has(okg, reg) ? gr[s0] : T(0);
     const T gvm1 = gvr[s4], gv0 = gvr[s5], gvp1 = gvr[s0];   // gv at y-edges y-4, y-3, y-2
+    T res = T(0), qx = T(0);                                  // the row's results, stored below
     // ---- column stage: parabola of row y-3, y-edge y-3, result row y-4
     if constexpr (STAGE >= 1) {
       T qL, qR;
       recon_edges<T, RECON>(fw[s1], fw[s2], fw[s3], fw[s4], fw[s5], qL, qR);
       const int r = y - 3;
-      T m3 = fw[s0];
+      T m3 = fw[s6];
       if constexpr (STAGE == 1 && RECON == 3) {
         // the first parabola row of a segment one row short of the north side (iend - 2): its
         // q_L override reads row r - 3, one before the march's first row, which no slot holds
-        if (pl07 && r == iend - 2) m3 = field(PASS == 1 ? lf : lg, r - 3, 1);
+        if (pl07 && r == iend - 2) m3 = PASS == 1 ? field(lf, okf, r - 3) : field(lg, okg, r - 3);
       }
       if (pl07 && ((r >= i0 - 1 && r <= i0 + 1) || (r >= iend - 2 && r <= iend)))
         pl07_y(r, m3, fw[s1], fw[s2], fw[s3], fw[s4], fw[s5], cur, qL, qR);
       const Parab<T> a = parab_of<T>(qL, qR, fw[s3], gv0, gvp1, gcw[s3], mt0);
       if constexpr (STAGE >= 2) {
-        const T v = in.v;
-        const bool vup = in.vs >= T(0);
+        const T v = vr[s0];
+        const bool vup = (RK2 ? vsr[s0] : v) >= T(0);
         const T cy = v * dtdy;
         const T g = upwind<T>(vup, pa_A, pa_B, pa_q6, a, cy, gv0, mt0);     // times dt/dy
         if constexpr (STAGE >= 3) {
@@ -241,23 +296,19 @@ struct GMarch {
               d1 = gv0 * cy;
               d2 = gvm1 * (vprev * dtdy);
             }
-            const T qy = split_update<T>(split, fw[s2], dG, gcw[s2], rgw[s2], d1, d2);
-            if (out_lane) st_at(Out, (uint32_t)(cell + own + ro * ld), qy);
+            res = split_update<T>(split, fw[s2], dG, gcw[s2], rgw[s2], d1, d2);   // Qy
           } else {
             // div = -(dF + dG) / (dt g);  Q - dt div = Q + (dF + dG) / g
             const T flx = dfw[s2] + dG;
-            if (out_lane) {
-              if (Div) {
-                const T dv = -flx / (dtc * gcw[s2]);
-                st_at(Div, (uint32_t)(own + ro * ld), dv);
-                part += (double)dv * (double)gcw[s2];
-              } else {
-                st_at(Out, (uint32_t)(own + ro * ld), qsw[s2] + flx * rgw[s2]);
-              }
-              if (fside) {   // MF-AF: the y-fluxes through the S (edge i0) and N (edge iend) sides
-                if (ro == i0) st_at(fside, (uint32_t)(3 * N + x - i0), gprev);
-                if (ro == iend - 1) st_at(fside, (uint32_t)(2 * N + x - i0), g);
-              }
+            if (Div) {
+              res = -flx / (dtc * gcw[s2]);
+              if (out_lane) part += (double)res * (double)gcw[s2];
+            } else {
+              res = qsw[s2] + flx * rgw[s2];
+            }
+            if (fside && out_lane) {   // MF-AF: the y-fluxes through the S (edge i0) and N (edge iend) sides
+              if (ro == i0) st_at(fside, (uint32_t)(3 * N + x - i0), gprev);
+              if (ro == iend - 1) st_at(fside, (uint32_t)(2 * N + x - i0), g);
             }
           }
         }
@@ -267,14 +318,13 @@ struct GMarch {
       pa_A = a.rA; pa_B = a.rB; pa_q6 = a.q6;
     }
     // ---- row stage: row y
-    const T f = in.f;
     T m2, m1, p1, p2;
     xwindow<T, false>(f, 0, 0, 0, 0, m2, m1, p1, p2);
     T qL, qR;
     recon_edges<T, RECON>(m2, m1, f, p1, p2, qL, qR);
     if (sp_x) pl07_x(m2, m1, f, p1, p2, qL, qR);
-    const T gu0 = guw[w0], gup1 = from_right(gu0), u = uw[w0];
-    const bool up = (RK2 ? usw[w0] : u) >= T(0);
+    const T gu0 = guw[s0], gup1 = from_right(gu0), u = uw[s0];
+    const bool up = (RK2 ? usw[s0] : u) >= T(0);
     const Parab<T> c = parab_of<T>(qL, qR, f, gu0, gup1, gcw[s0], mt0);
     const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);     // left edge of the lane's cell
     const T dF = f1 - from_right(f1);
@@ -285,8 +335,7 @@ struct GMarch {
         c1 = gup1 * (from_right(u) * dtdx);
         c2 = gu0 * (u * dtdx);
       }
-      const T qx = split_update<T>(split, f, dF, gcw[s0], rgw[s0], c1, c2);
-      if (out_lane && row_out) st_at(Out, (uint32_t)(own + y * ld), qx);
+      qx = split_update<T>(split, f, dF, gcw[s0], rgw[s0], c1, c2);
     } else {
       dfw[s0] = dF;
       if (fside && row_out) {      // MF-AF: the x-fluxes through the W (edge i0) and E (iend) sides
@@ -294,33 +343,52 @@ struct GMarch {
         if (e_lane) st_at(fside, (uint32_t)(0 * N + y - i0), f1);
       }
     }
-    // row y replaces row y-6 in the ring of the column stage's field
+    // row y replaces row y-RD in the ring of the column stage's field
     fw[s0] = cur;
-    load<(K + PF) % 6, (K + PF) % 3>(y + PF, in);
+    load<sl(K, PF)>(y + PF);
+    // the row's stores (byte offsets within out; the first rows issue them too, dropped)
+    constexpr uint32_t sz = sizeof(T);
+    const int ro = y - 4;
+    if constexpr (PASS == 1) {
+      st_buf(out, STAGE >= 3 && out_lane ? (uint32_t)(cell + own + ro * ld) * sz : kNoStore, res);   // Qy, row y-4
+      st_buf(out, out_lane && row_out ? (uint32_t)(own + y * ld) * sz : kNoStore, qx);               // Qx, row y
+    } else {
+      st_buf(out, STAGE >= 3 && out_lane ? (uint32_t)(own + ro * ld) * sz : kNoStore, res);
+    }
   }
 
-  __device__ __forceinline__ void run(GIn<T> *b) {
+  // row y0 + K0 + j in phase (K0 + j) % RD
+  template <int K0, int STAGE, int J>
+  __device__ __forceinline__ void row(int y) {
+    body<(K0 + J) % RD, STAGE>(y + J);
+  }
+  template <int K0, int... J>
+  __device__ __forceinline__ void full_rows(int y, std::integer_sequence<int, J...>) {
+    (row<K0, 3, J>(y), ...);
+  }
+  template <int K0, int... J>
+  __device__ __forceinline__ void last_rows(int y, std::integer_sequence<int, J...>) {
+    ((y + J <= ylast ? row<K0, 3, J>(y) : void()), ...);
+  }
+
+  __device__ __forceinline__ void run() {
 #pragma unroll
-    for (int k = 0; k < 6; ++k) fw[k] = dfw[k] = qsw[k] = gcw[k] = rgw[k] = gvr[k] = T(0);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) uw[k] = usw[k] = guw[k] = T(0);
+    for (int k = 0; k < RD; ++k)
+      fr[k] = gr[k] = vr[k] = vsr[k] = fw[k] = dfw[k] = qsw[k] = gcw[k] = rgw[k] = gvr[k] = uw[k] = usw[k] = guw[k] = T(0);
     gprev = vprev = pa_A = pa_B = pa_q6 = T(0);
     part = 0.0;
+    lrow = -1000;
+    lreg = -1;
+    of = og = oq = bf = bg = 0;
     int y = tj - 3;
-    load<0, 0>(y, b[0]);
-    load<1, 1>(y + 1, b[1]);
+    load<0>(y);
+    load<1>(y + 1);
     // rows y0 .. y0+6 (ylast - y0 = ny + 6 >= 7: they all exist), the column stage starting
-    body<0, 0>(y, b[0]); body<1, 0>(y + 1, b[1]); body<2, 0>(y + 2, b[0]); body<3, 0>(y + 3, b[1]);
-    body<4, 0>(y + 4, b[0]); body<5, 1>(y + 5, b[1]); body<0, 2>(y + 6, b[0]);
-    for (y += 7; y + 5 <= ylast; y += 6) {
-      body<1, 3>(y, b[1]); body<2, 3>(y + 1, b[0]); body<3, 3>(y + 2, b[1]);
-      body<4, 3>(y + 3, b[0]); body<5, 3>(y + 4, b[1]); body<0, 3>(y + 5, b[0]);
-    }
-    if (y <= ylast) body<1, 3>(y, b[1]);
-    if (y + 1 <= ylast) body<2, 3>(y + 1, b[0]);
-    if (y + 2 <= ylast) body<3, 3>(y + 2, b[1]);
-    if (y + 3 <= ylast) body<4, 3>(y + 3, b[0]);
-    if (y + 4 <= ylast) body<5, 3>(y + 4, b[1]);
+    row<0, 0, 0>(y); row<0, 0, 1>(y); row<0, 0, 2>(y); row<0, 0, 3>(y);
+    row<0, 0, 4>(y); row<0, 1, 5>(y); row<0, 2, 6>(y);
+    // rows y0+7 on: RD rows per iteration
+    for (y += 7; y + RD - 1 <= ylast; y += RD) full_rows<7>(y, std::make_integer_sequence<int, RD>());
+    last_rows<7>(y, std::make_integer_sequence<int, RD - 1>());
   }
 };
 
@@ -346,7 +414,7 @@ __device__ __forceinline__ bool gp_running(const csadv_solve_desc &D, int run) {
   return run >= 0 && !(D.stop > 0 && run >= D.stop);
 }
 
-template <typename T, int RECON, bool RK2, int PASS>
+template <typename T, int RECON, bool RK2, int PASS, int SPLIT>
 __device__ __forceinline__ void gp_item(const Solve &S, int tr, int p, int sx, int sg, bool last) {
   const csadv_solve_desc &D = S.d;
   const int lane = threadIdx.x;
@@ -367,7 +435,7 @@ __device__ __forceinline__ void gp_item(const Solve &S, int tr, int p, int sx, i
   const bool odd = run & 1;                   // the state of step n lives in q[n & 1]
   const int64_t toff = tr * S.tstride;
   T *qin = reinterpret_cast<T *>(odd ? D.q1 : D.q0) + toff, *qout = reinterpret_cast<T *>(odd ? D.q0 : D.q1) + toff;
-  GMarch<T, RECON, RK2, PASS> m;
+  GMarch<T, RECON, RK2, PASS, SPLIT> m;
   m.ld = ld;
   m.ldu = D.ldu;
   m.N = N;
@@ -380,9 +448,8 @@ __device__ __forceinline__ void gp_item(const Solve &S, int tr, int p, int sx, i
   m.tend = tj + ny;
   m.ylast = tj + ny + 3;
   m.cell = (int)w.cell;
-  m.own = p * ps + m.x;
-  m.split = D.split;
-  m.mt0 = D.split != 3;
+  // (the clamped column: the lanes past the array's last column read Q in bounds)
+  m.own = p * ps + m.xc;
   m.pl07 = D.et == 2;
   m.out_lane = lane >= 3 && lane < 3 + nx;
   m.w_lane = ti == NG && lane == 3;
@@ -406,25 +473,30 @@ __device__ __forceinline__ void gp_item(const Solve &S, int tr, int p, int sx, i
   m.gv = reinterpret_cast<const T *>(D.gv);
   m.rgc = reinterpret_cast<const T *>(D.rgc);
   m.Qs = qin;
-  m.Div = nullptr;
+  m.Div = false;
   m.fside = nullptr;
   const int cell = m.cell;
+  const uint32_t cell_bytes = (uint32_t)(w.cell * sizeof(T));
+  m.lg = Lines{0, 0, 0, 0, 0, 0, 0u};
   if constexpr (PASS == 1) {
     m.X = qin;
-    m.Out = work + w.o_qx;
+    m.out = buffer_of(work + w.o_qx, 2 * cell_bytes);
     m.lf = lane_lines(D, ps, p, m.x, 0, true, 0, true, 0);
   } else {
     m.X = work + w.o_qx;
-    m.Out = qout;
-    if (D.mf == 3) m.Div = work + w.o_div;
+    m.Div = D.mf == 3;
+    m.out = buffer_of(m.Div ? work + w.o_div : qout, cell_bytes);
     if (D.mf == 2) m.fside = work + w.o_side + p * 4 * N;
     // the row stage's Qy (own at + cell; E/W ghosts from the neighbour's Qx on panels 4, 5)
     m.lf = lane_lines(D, ps, p, m.x, cell, true, (p == 4 || p == 5) ? 0 : cell, false, 0);
     // the column stage's Qx (N/S ghosts from the neighbour's Qy on panels 1, 3)
     m.lg = lane_lines(D, ps, p, m.x, 0, false, 0, true, (p == 1 || p == 3) ? cell : 0);
   }
-  GIn<T> b[2];
-  m.run(b);
+  for (int k = 0; k < 3; ++k) {
+    m.okf[k] = (m.lf.ok >> k) & 1u;
+    m.okg[k] = (m.lg.ok >> k) & 1u;
+  }
+  m.run();
   if constexpr (PASS == 2) {
     if (D.mf == 3) {
       const double part = warp_sum(m.part);
@@ -437,11 +509,11 @@ __device__ __forceinline__ void gp_item(const Solve &S, int tr, int p, int sx, i
 
 // One tracer's pass of one solve; the descriptor travels in the kernel arguments.  last: this
 // pass ends the step (pass 2 of the last tracer with neither MF-AF nor MF-PR after it).
-template <typename T, int RECON, bool RK2, int PASS>
+template <typename T, int RECON, bool RK2, int PASS, int SPLIT>
 __global__ __launch_bounds__(WAVE) void k_gp_pass(const Solve one, int tr, int last) {
   int p, sx, sg;
   if (!gen_item(one, touch_solve(one, blockIdx.x), p, sx, sg)) return;
-  gp_item<T, RECON, RK2, PASS>(one, tr, p, sx, sg, last != 0);
+  gp_item<T, RECON, RK2, PASS, SPLIT>(one, tr, p, sx, sg, last != 0);
 }
 
 // MF-AF: the cube-edge averages of the second pass's fluxes into the boundary cells of the new
@@ -508,7 +580,7 @@ inline int gen_blocks(const Solve &S) { return kXcds * ((S.segs + kXcds - 1) / k
 
 // One step of every solve by the general step: per tracer the two passes, then the mass fixer's
 // kernels; the last kernel of the last tracer advances the step counter.
-template <typename T, int R, bool RK2>
+template <typename T, int R, bool RK2, int SP>
 int enqueue_general_k(csadv_t *h, hipStream_t st) {
   for (size_t s = 0; s < h->host.size(); ++s) {
     const Solve &S = h->host[s];
@@ -516,8 +588,8 @@ int enqueue_general_k(csadv_t *h, hipStream_t st) {
     const int blocks = gp::gen_blocks(S);
     for (int tr = 0; tr < d.ntr; ++tr) {
       const int last = tr == d.ntr - 1;
-      hipLaunchKernelGGL((gp::k_gp_pass<T, R, RK2, 1>), dim3(blocks), dim3(WAVE), 0, st, S, tr, 0);
-      hipLaunchKernelGGL((gp::k_gp_pass<T, R, RK2, 2>), dim3(blocks), dim3(WAVE), 0, st, S, tr,
+      hipLaunchKernelGGL((gp::k_gp_pass<T, R, RK2, 1, SP>), dim3(blocks), dim3(WAVE), 0, st, S, tr, 0);
+      hipLaunchKernelGGL((gp::k_gp_pass<T, R, RK2, 2, SP>), dim3(blocks), dim3(WAVE), 0, st, S, tr,
                          last && d.mf == 1);
       if (d.mf == 2) {
         const int cells = 6 * 4 * (d.N - 1);
@@ -533,17 +605,17 @@ int enqueue_general_k(csadv_t *h, hipStream_t st) {
   return CSADV_OK;
 }
 
-template <typename T, int R>
+template <typename T, int R, int SP>
 int enqueue_general_r(csadv_t *h, hipStream_t st) {
-  return h->rk2 ? enqueue_general_k<T, R, true>(h, st) : enqueue_general_k<T, R, false>(h, st);
+  return h->rk2 ? enqueue_general_k<T, R, true, SP>(h, st) : enqueue_general_k<T, R, false, SP>(h, st);
 }
 
-template <typename T>
-int enqueue_general_t(csadv_t *h, hipStream_t st) {
+template <typename T, int SP>
+int enqueue_general_split(csadv_t *h, hipStream_t st) {
   switch (h->recon) {
-    case 1: return enqueue_general_r<T, 1>(h, st);
-    case 2: return enqueue_general_r<T, 2>(h, st);
-    case 4: return enqueue_general_r<T, 4>(h, st);
-    default: return enqueue_general_r<T, 3>(h, st);
+    case 1: return enqueue_general_r<T, 1, SP>(h, st);
+    case 2: return enqueue_general_r<T, 2, SP>(h, st);
+    case 4: return enqueue_general_r<T, 4, SP>(h, st);
+    default: return enqueue_general_r<T, 3, SP>(h, st);
   }
 }

@@ -67,6 +67,13 @@ int ghost_fill(csadv_t *h, hipStream_t st);
 template <typename T, int R>
 hipError_t launch_step_r(csadv_t *h, hipStream_t st);
 
-// One general step of every solve (general_<dtype>.hip).
+// One general step of every solve with splitting SP (general_<dtype>_s<SP>.hip), and by the
+// handle's splitting.
+template <typename T, int SP>
+int enqueue_general_split(csadv_t *h, hipStream_t st);
 template <typename T>
-int enqueue_general_t(csadv_t *h, hipStream_t st);
+inline int enqueue_general_t(csadv_t *h, hipStream_t st) {
+  if (h->split == 2) return enqueue_general_split<T, 2>(h, st);
+  if (h->split == 3) return enqueue_general_split<T, 3>(h, st);
+  return enqueue_general_split<T, 1>(h, st);
+}

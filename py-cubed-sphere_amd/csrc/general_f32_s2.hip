@@ -1,0 +1,5 @@
+// The general step (ET-S72, ET-PL07, MF-AF) of FP32 with the L04 splitting: one translation unit
+// per precision and splitting, so that the library's kernels compile in parallel.
+#include "csadv_general.hpp"
+
+template int enqueue_general_split<float, 2>(csadv_t *h, hipStream_t st);
