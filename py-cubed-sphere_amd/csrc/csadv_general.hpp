@@ -9,7 +9,7 @@
 //   MF-PR: k_gp_prsum sums the partials, k_gp_update applies the projection and the update.
 //
 // Each pass marches one wavefront down a column strip as the fused step does (lane = column,
-// the 3-cell halo in the outer lanes, x windows by DPP, a six-row ring for the y windows).  No
+// the 3-cell halo in the outer lanes, x windows by DPP, register rings for the y windows).  No
 // ghost cell is stored: a lane in a ghost column (interior rows) or a row in a ghost row
 // (interior columns) reads the neighbour panel's cell at the same depth through the halo
 // orientation table, from the field the copy halo would have copied there -- Q in pass 1; in

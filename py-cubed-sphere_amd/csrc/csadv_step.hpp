@@ -11,7 +11,7 @@
 // Work item = (solve, panel, tracer, strip, segment): one wavefront marches down strip_w interior
 // columns (lane l <-> column ti-3+l, so the 3-cell halo of each side rides in the outer
 // lanes) over seg_l interior rows.  Iteration y:
-//   * column stage at y-edge e = y-3 from the 6-row windows (rows y-6 .. y-1): y-fluxes of
+//   * column stage at y-edge e = y-3 from the windows of rows y-5 .. y-1: y-fluxes of
 //     Q and of Qx, Qy of row y-4, x-flux of Qy (neighbours by DPP lane shifts), update of
 //     row y-4 and its mass-fixer partial sum;
 //   * row stage for row y: x-flux of Q, split half-step Qx.
