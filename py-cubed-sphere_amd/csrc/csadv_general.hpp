@@ -132,8 +132,10 @@ struct GMarch {
   // ---- per wavefront
   const T *X;              // pass 1: Q of the tracer; pass 2: Qx of the tracer (Qy = Qx + cell)
   const T *Qs;             // pass 2: Q of the tracer (the update's input)
-  // pass 1: Qx and Qy of the tracer ([2][cell]); pass 2: the new Q, or with MF-PR its div
-  __amdgpu_buffer_rsrc_t out;
+  // pass 1: Qx (out) and Qy (outy) of the tracer; pass 2: the new Q, or with MF-PR its div
+  // (out).  Each covers one cell array, under 2^31 bytes up to the largest N csadv_create
+  // accepts for the general step, so that kNoStore lies past every one of them.
+  __amdgpu_buffer_rsrc_t out, outy;
   bool Div;                // pass 2: out holds div (MF-PR; the update follows later)
   T *fside;                // pass 2 with MF-AF: cube-edge fluxes of the panel, [E, W, N, S][N]
   const T *U, *US, *V, *VS, *gc, *gu, *gv, *rgc;
@@ -350,7 +352,7 @@ struct GMarch {
     constexpr uint32_t sz = sizeof(T);
     const int ro = y - 4;
     if constexpr (PASS == 1) {
-      st_buf(out, STAGE >= 3 && out_lane ? (uint32_t)(cell + own + ro * ld) * sz : kNoStore, res);   // Qy, row y-4
+      st_buf(outy, STAGE >= 3 && out_lane ? (uint32_t)(own + ro * ld) * sz : kNoStore, res);   // Qy, row y-4
       st_buf(out, out_lane && row_out ? (uint32_t)(own + y * ld) * sz : kNoStore, qx);               // Qx, row y
     } else {
       st_buf(out, STAGE >= 3 && out_lane ? (uint32_t)(own + ro * ld) * sz : kNoStore, res);
@@ -480,7 +482,8 @@ __device__ __forceinline__ void gp_item(const Solve &S, int tr, int p, int sx, i
   m.lg = Lines{0, 0, 0, 0, 0, 0, 0u};
   if constexpr (PASS == 1) {
     m.X = qin;
-    m.out = buffer_of(work + w.o_qx, 2 * cell_bytes);
+    m.out = buffer_of(work + w.o_qx, cell_bytes);
+    m.outy = buffer_of(work + w.o_qy, cell_bytes);
     m.lf = lane_lines(D, ps, p, m.x, 0, true, 0, true, 0);
   } else {
     m.X = work + w.o_qx;

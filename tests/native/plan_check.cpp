@@ -105,7 +105,15 @@ int main() {
     const bool fits = general_pair_bytes(N, g.dtype) < (int64_t(1) << 32);
     const int grc = validate(g, msg);
     CHECK(fits ? grc == CSADV_OK : grc == CSADV_E_RANGE, "general N=%d dtype %d: %d %s", N, g.dtype, grc, msg.c_str());
-    ++cases;
+    // the passes' stores go through buffers of one cell array each, whose byte size must stay
+    // below the dropped-store offset (2^31) wherever the general step is accepted
+    for (int dt : {CSADV_F64, CSADV_F32}) {
+      g.dtype = dt;
+      const int64_t cell_bytes = int64_t(6) * (N + 2 * NG) * general_ld(N) * (dt == CSADV_F64 ? 8 : 4);
+      if (validate(g, msg) == CSADV_OK) CHECK(cell_bytes < (int64_t(1) << 31), "general N=%d dtype %d: cell array of %lld bytes", N, dt, (long long)cell_bytes);
+      ++cases;
+    }
+    g.dtype = d.dtype;
   }
   // ---- handle compatibility
   {
