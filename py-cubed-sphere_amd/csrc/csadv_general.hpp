@@ -108,8 +108,9 @@ template <typename T> __device__ __forceinline__ T pl_right(T m2, T m1, T c0, T 
 //   * column stage at y-edge y-3 from the ring of the y-field (rows y-6 .. y-1): the
 //     parabola of row y-3, the y-flux through edge y-3, then for row y-4 the y-split Qy
 //     (pass 1) or the update (pass 2, with the x-flux difference the row stage kept);
-//   * row stage for row y: x-window of the x-field by DPP, parabolas, x-fluxes (every lane the
-//     flux through its cell's left edge), then Qx (pass 1) or the x-flux difference (pass 2).
+//   * row stage for row y, in the segment's own rows tj .. tj+ny-1: x-window of the x-field by
+//     DPP, parabolas, x-fluxes (every lane the flux through its cell's left edge), then Qx
+//     (pass 1) or the x-flux difference (pass 2).
 // The inputs of row y+PF are requested at the end of iteration y, PF = 2 rows ahead.  Loads and
 // stores share one counter, and a wait for a load also waits for everything issued before it,
 // so the march keeps that count exact: every per-row value sits in a ring one slot deeper than
@@ -319,30 +320,35 @@ struct GMarch {
       }
       pa_A = a.rA; pa_B = a.rB; pa_q6 = a.q6;
     }
-    // ---- row stage: row y
-    T m2, m1, p1, p2;
-    xwindow<T, false>(f, 0, 0, 0, 0, m2, m1, p1, p2);
-    T qL, qR;
-    recon_edges<T, RECON>(m2, m1, f, p1, p2, qL, qR);
-    if (sp_x) pl07_x(m2, m1, f, p1, p2, qL, qR);
-    const T gu0 = guw[s0], gup1 = from_right(gu0), u = uw[s0];
-    const bool up = (RK2 ? usw[s0] : u) >= T(0);
-    const Parab<T> c = parab_of<T>(qL, qR, f, gu0, gup1, gcw[s0], mt0);
-    const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);     // left edge of the lane's cell
-    const T dF = f1 - from_right(f1);
+    // ---- row stage: row y.  Its result (Qx, or the x-flux difference) is used only in the
+    // segment's own rows: the halo rows of the march feed the column stage alone, since each pass
+    // reads the other direction's half-step from the work buffer, not from this march.  So the
+    // ny + 7 iterations of a segment run it ny times (a wave-uniform branch).
     const bool row_out = y >= tj && y < tend;
-    if constexpr (PASS == 1) {
-      T c1 = T(0), c2 = T(0);
-      if (split != 1) {
-        c1 = gup1 * (from_right(u) * dtdx);
-        c2 = gu0 * (u * dtdx);
-      }
-      qx = split_update<T>(split, f, dF, gcw[s0], rgw[s0], c1, c2);
-    } else {
-      dfw[s0] = dF;
-      if (fside && row_out) {      // MF-AF: the x-fluxes through the W (edge i0) and E (iend) sides
-        if (w_lane) st_at(fside, (uint32_t)(1 * N + y - i0), f1);
-        if (e_lane) st_at(fside, (uint32_t)(0 * N + y - i0), f1);
+    if (row_out) {
+      T m2, m1, p1, p2;
+      xwindow<T, false>(f, 0, 0, 0, 0, m2, m1, p1, p2);
+      T qL, qR;
+      recon_edges<T, RECON>(m2, m1, f, p1, p2, qL, qR);
+      if (sp_x) pl07_x(m2, m1, f, p1, p2, qL, qR);
+      const T gu0 = guw[s0], gup1 = from_right(gu0), u = uw[s0];
+      const bool up = (RK2 ? usw[s0] : u) >= T(0);
+      const Parab<T> c = parab_of<T>(qL, qR, f, gu0, gup1, gcw[s0], mt0);
+      const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);     // left edge of the lane's cell
+      const T dF = f1 - from_right(f1);
+      if constexpr (PASS == 1) {
+        T c1 = T(0), c2 = T(0);
+        if (split != 1) {
+          c1 = gup1 * (from_right(u) * dtdx);
+          c2 = gu0 * (u * dtdx);
+        }
+        qx = split_update<T>(split, f, dF, gcw[s0], rgw[s0], c1, c2);
+      } else {
+        dfw[s0] = dF;
+        if (fside) {               // MF-AF: the x-fluxes through the W (edge i0) and E (iend) sides
+          if (w_lane) st_at(fside, (uint32_t)(1 * N + y - i0), f1);
+          if (e_lane) st_at(fside, (uint32_t)(0 * N + y - i0), f1);
+        }
       }
     }
     // row y replaces row y-RD in the ring of the column stage's field
