@@ -217,22 +217,26 @@ struct March {
       pa_A = a.rA; pa_B = a.rB; pa_q6 = a.q6;
       pb_A = b.rA; pb_B = b.rB; pb_q6 = b.q6;
     }
-    // ---- row stage: row y
-    const T u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
-    const bool up = wind_sign(s0) >= T(0);
-    const T gup1 = from_right(gu0);
-    T q_m2, q_m1, q_p1, q_p2;
-    xwindow<T, XPULL>(q, am2, am1, ap1, ap2, q_m2, q_m1, q_p1, q_p2);
-    const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
-    const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);
-    const T dF1 = f1 - from_right(f1);
-    T c1 = T(0), c2 = T(0);
-    if (split != 1) {
-      c1 = gup1 * (from_right(u) * dtdx);
-      c2 = gu0 * (u * dtdx);
+    // ---- row stage: row y.  Qx of the march's last row, ylast, is read by no stencil (the
+    // last parabola row is ylast-3, whose window ends at ylast-1).  FP32, bound by issue, skips
+    // it by a wave-uniform branch (2-3% of the launch); FP64, bound by latency, gains nothing
+    if (!std::is_same<T, float>::value || STAGE < 3 || y < ylast) {
+      const T u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
+      const bool up = wind_sign(s0) >= T(0);
+      const T gup1 = from_right(gu0);
+      T q_m2, q_m1, q_p1, q_p2;
+      xwindow<T, XPULL>(q, am2, am1, ap1, ap2, q_m2, q_m1, q_p1, q_p2);
+      const Parab<T> c = parabola<T, RECON>(q_m2, q_m1, q, q_p1, q_p2, gu0, gup1, gc0, mt0);
+      const T f1 = upwind_x<T>(up, c, u * dtdx, gu0, gup1, mt0);
+      const T dF1 = f1 - from_right(f1);
+      T c1 = T(0), c2 = T(0);
+      if (split != 1) {
+        c1 = gup1 * (from_right(u) * dtdx);
+        c2 = gu0 * (u * dtdx);
+      }
+      xw[s0] = split_update<T>(split, q, dF1, gc0, rgw[s0], c1, c2);
     }
-    const T qx = split_update<T>(split, q, dF1, gc0, rgw[s0], c1, c2);
-    qw[s0] = q; xw[s0] = qx;
+    qw[s0] = q;
     if constexpr (std::is_same<T, float>::value) sw[s0] = raw;
     // rows past ylast are read in bounds (clamped rows, masked Q) and never used
     load<false, sl(K, PF)>(y + PF);
