@@ -219,6 +219,23 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert len(rows) >= 6 and len(set(rows)) == len(rows)
 
 
+def test_build_flags_of_the_device_units():
+    """Every device unit is built for gfx950 without packed FP32 arithmetic; the FP32 fused-step
+    units fuse multiply-adds only within one source expression (their one-solve and multi-solve
+    kernels, a solve alone and in a batch, must round alike whatever their code layout), and no
+    other unit sets the contraction mode."""
+    from cubedsphere import build
+    cmds = {os.path.basename(c[-1]): c for c in build.compile_commands()}
+    hip = {n: c for n, c in cmds.items() if n.endswith(".hip")}
+    assert any(n.startswith("step_f32_") for n in hip) and any(n.startswith("step_f64_") for n in hip)
+    for name, cmd in hip.items():
+        assert "--offload-arch=gfx950" in cmd, name
+        k = cmd.index("-packed-fp32-ops")
+        assert cmd[k - 3:k] == ["-Xclang", "-target-feature", "-Xclang"], name
+        contract = [f for f in cmd if f.startswith("-ffp-contract")]
+        assert contract == (["-ffp-contract=on"] if name.startswith("step_f32_") else []), name
+
+
 def test_descriptor_struct_matches_header_field_order():
     from cubedsphere import _lib
     header = open(os.path.join(ROOT, "include", "csadv.h")).read()
