@@ -219,7 +219,9 @@ struct March {
     }
     // ---- row stage: row y.  Qx of the march's last row, ylast, is read by no stencil (the
     // last parabola row is ylast-3, whose window ends at ylast-1).  FP32, bound by issue, skips
-    // it by a wave-uniform branch (2-3% of the launch); FP64, bound by latency, gains nothing
+    // it by a wave-uniform branch (2-3% of the launch).  FP64 does not: the branch changes the
+    // code layout, and the one-solve and multi-solve kernels then round alike only when
+    // multiply-adds are fused per expression, which costs the latency-bound FP64 kernel 1.5-2%
     if (!std::is_same<T, float>::value || STAGE < 3 || y < ylast) {
       const T u = uw[s0], gu0 = guw[s0], gc0 = gcw[s0];
       const bool up = wind_sign(s0) >= T(0);
