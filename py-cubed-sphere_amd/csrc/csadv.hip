@@ -897,7 +897,9 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
   }
   // workgroup b runs on XCD b % kXcds (the list interleaves one band of row segments per XCD)
   static_assert(sizeof(Item) == sizeof(int4), "work items are read as int4");
-  const std::vector<Item> work = work_list(h->host);
+  // (the FP32 multi-solve step kernel marches two tracers per item: kStepGroup in csadv_step.hpp)
+  const int group = (!h->general && h->dtype == CSADV_F32 && nsolve > 1) ? 2 : 1;
+  const std::vector<Item> work = work_list(h->host, group);
   if ((int)work.size() != h->nwork) {
     if (h->work) (void)hipFree(h->work);
     h->work = nullptr;

@@ -97,18 +97,19 @@ Solve derive(const csadv_solve_desc &d, int tile) {
   return s;
 }
 
-std::vector<Item> work_list(const std::vector<Solve> &solves) {
+std::vector<Item> work_list(const std::vector<Solve> &solves, int group) {
   // segment rows are cut into kXcds contiguous bands, one per XCD: the band's halo rows and
   // its share of the panel-invariant metric then stay in that XCD's L2 for all six panels
   // (the tracers of one segment are neighbours in a band, so they share its wind and metric
-  // rows in L2; the work item packs panel and tracer as p + 8 * tracer)
+  // rows in L2; the work item packs panel and tracer as p + 8 * tracer, the first tracer of a
+  // group of `group` consecutive tracers, the last group of a solve holding the rest)
   std::vector<std::vector<Item>> band(kXcds);
   for (int s = 0; s < (int)solves.size(); ++s) {
     const Solve &S = solves[s];
     for (int sg = 0; sg < S.segs; ++sg)
       for (int p = 0; p < 6; ++p)
         for (int sx = 0; sx < S.strips; ++sx)
-          for (int tr = 0; tr < S.d.ntr; ++tr) band[sg * kXcds / S.segs].push_back(Item{s, p + 8 * tr, sx, sg});
+          for (int tr = 0; tr < S.d.ntr; tr += group) band[sg * kXcds / S.segs].push_back(Item{s, p + 8 * tr, sx, sg});
   }
   size_t blen = 0;
   for (auto &bd : band) blen = std::max(blen, bd.size());

@@ -126,11 +126,12 @@ void tiling(int N, int tile, int &strip_w, int &strips, int &seg_l, int &segs);
 // The descriptor with its derived strides; tile = the handle's tile when d.tile < 0.
 Solve derive(const csadv_solve_desc &d, int tile);
 
-// The step kernel's work list: every (solve, panel, tracer, strip, segment) once, the row
-// segments of each solve cut into kXcds contiguous bands, one per XCD, and the bands
+// The step kernel's work list: every (solve, panel, tracer group, strip, segment) once, the
+// row segments of each solve cut into kXcds contiguous bands, one per XCD, and the bands
 // interleaved so that workgroup b (on XCD b % kXcds) takes its item from band b % kXcds
-// (padded with solve -1 to equal length).
-std::vector<Item> work_list(const std::vector<Solve> &solves);
+// (padded with solve -1 to equal length).  group: tracers per item (the FP32 multi-solve
+// step kernel marches two).
+std::vector<Item> work_list(const std::vector<Solve> &solves, int group = 1);
 
 // Sizes of the public entry points (see include/csadv.h).
 int partials_per_tracer(int N, int tile);

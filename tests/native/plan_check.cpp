@@ -143,7 +143,8 @@ int main() {
   CHECK(tile_rows(-1) < 0 && tile_rows(kNumTiles) < 0 && tile_rows(0) == kSegRows[0], "tile_rows");
   // ---- work lists of random batches
   std::mt19937 rng(2024);
-  for (int trial = 0; trial < 60; ++trial) {
+  for (int trial = 0; trial < 120; ++trial) {
+    const int group = 1 + trial % 2;             // tracers per item (FP64 1, FP32 fused step 2)
     const int nsolve = 1 + (int)(rng() % 70);
     std::vector<Solve> solves;
     for (int s = 0; s < nsolve; ++s) {
@@ -152,12 +153,12 @@ int main() {
       d.tile = (int)(rng() % (kNumTiles + 1)) - 1;
       solves.push_back(derive(d, (int)(rng() % kNumTiles)));
     }
-    const std::vector<Item> work = work_list(solves);
+    const std::vector<Item> work = work_list(solves, group);
     CHECK(work.size() % kXcds == 0, "work list length %zu", work.size());
     std::set<std::tuple<int, int, int, int>> seen;
     std::map<std::pair<int, int>, int> band_of;   // (solve, segment) -> XCD
     int64_t expect = 0;
-    for (auto &S : solves) expect += (int64_t)6 * S.strips * S.segs * S.d.ntr;
+    for (auto &S : solves) expect += (int64_t)6 * S.strips * S.segs * ((S.d.ntr + group - 1) / group);
     int64_t real = 0;
     for (size_t b = 0; b < work.size(); ++b) {
       const Item &it = work[b];
@@ -166,7 +167,7 @@ int main() {
       CHECK(it.x < nsolve, "solve index");
       const Solve &S = solves[it.x];
       const int p = it.y & 7, tr = it.y >> 3;
-      CHECK(p < 6 && tr < S.d.ntr && it.z < S.strips && it.w < S.segs, "item out of range");
+      CHECK(p < 6 && tr < S.d.ntr && tr % group == 0 && it.z < S.strips && it.w < S.segs, "item out of range");
       CHECK(seen.insert({it.x, it.y, it.z, it.w}).second, "item listed twice");
       const int xcd = (int)(b % kXcds);
       auto key = std::make_pair(it.x, it.w);
