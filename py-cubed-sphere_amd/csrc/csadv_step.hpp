@@ -450,7 +450,8 @@ __device__ __forceinline__ void step_item(const Solve &S, const int4 wk) {
 // with an odd tracer count holds one), one in FP64.  Measured against one tracer per item
 // (same box, 64 seeded solves with 256 tracers, 16-row tiles): 565 -> 554 us per launch.  The
 // one-solve kernel keeps one tracer per item: its wavefronts drop from 5 to 3 per SIMD with
-// two (94 -> 129 VGPRs), and N=768 with 4 tracers ran 67.7 -> 69.0 us at its best tile.
+// two (94 -> 129 VGPRs), and N=768 with 4 tracers ran 67.7 -> 69.0 us at its best tile; with
+// two and DPP windows (4 per SIMD) its whole step gained nothing.
 template <typename T> constexpr int kStepGroup = std::is_same<T, float>::value ? 2 : 1;
 
 // One item of NT = kStepGroup<T> tracers, or of one where the solve has no second tracer left
