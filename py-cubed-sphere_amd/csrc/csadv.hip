@@ -162,8 +162,9 @@ __device__ __forceinline__ double corr_total(double dt, double a2, double c0, do
   return pending ? prev + dt * sum / a2 : held;
 }
 
+// blk: the block of BLOCK ghost cells, or -1 for the tracer's correction-total block; tr: tracer.
 template <typename T>
-__device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
+__device__ __forceinline__ void ghost_cells(const Solve &S, double *red, int blk, int tr) {
   const csadv_solve_desc &D = S.d;
   // Every descriptor field of this kernel is requested up front: the empty asm below takes them
   // all, so their scalar loads are issued together before the first branch (one round trip,
@@ -180,18 +181,16 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
   const int32_t *const strip_src = D.strip_src, *const strip_tgt = D.strip_tgt;
   const int32_t *const corner_src = D.corner_src, *const corner_tgt = D.corner_tgt;
   const double *const strip_w = D.strip_w, *const corner_w = D.corner_w;
-  const unsigned last_block = gridDim.x - 1;
-  // one tracer per grid layer (tracers run side by side); the tracer index passes through the
-  // asm (unchanged), so it exists only once every field is in registers
-  int tr = blockIdx.z;
+  // the tracer index passes through the asm (unchanged), so it exists only once every field is
+  // in registers
   asm("" : "=s"(tr) : "0"(tr), "s"(ntr), "s"(mf), "s"(stop), "s"(n_strip), "s"(n_corner), "s"(part_n),
       "s"(tstride), "s"(dt), "s"(a2), "s"(ctl), "s"(corr), "s"(partials), "s"(q0), "s"(q1), "s"(strip_src),
-      "s"(strip_tgt), "s"(corner_src), "s"(corner_tgt), "s"(strip_w), "s"(corner_w), "s"(last_block));
+      "s"(strip_tgt), "s"(corner_src), "s"(corner_tgt), "s"(strip_w), "s"(corner_w), "s"(blk));
   if (tr >= ntr) return;
   const int steps = ctl[1];            // requested before the recipes, whose wait it would join
   const int halted = ctl[0];           // set by a diagnostics call whose blow-up limit was exceeded
   const bool done = (stop > 0 && steps >= stop) || halted != 0;
-  if (blockIdx.x == last_block) {      // the correction total after the last step
+  if (blk < 0) {                       // the correction total after the last step
     if (mf != 3) return;
     // the partials and both stored totals are requested with the control words, not after
     // them: no load of this block waits for another
@@ -203,8 +202,8 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
     return;
   }
   const int ncell = n_strip + n_corner;
-  if ((int)(blockIdx.x * BLOCK) >= ncell && blockIdx.x != 0) return;
-  const int idx = blockIdx.x * BLOCK + threadIdx.x;
+  if (blk * BLOCK >= ncell && blk != 0) return;
+  const int idx = blk * BLOCK + (int)threadIdx.x;
   const bool active = idx < ncell;
   const bool strip = idx < n_strip;
   const int c = strip ? idx : idx - n_strip;
@@ -236,7 +235,7 @@ __device__ __forceinline__ void ghost_cells(const Solve &S, double *red) {
       tgt = corner_tgt[c];
     }
   }
-  const bool lead = blockIdx.x == 0 && tr == 0 && threadIdx.x == 0;
+  const bool lead = blk == 0 && tr == 0 && threadIdx.x == 0;
   if (done) {       // solve finished (k_step reads ctl[2] and skips it too) or halted (ctl[2] < 0)
     if (lead) ctl[2] = halted ? -1 : steps;
     return;
@@ -279,17 +278,20 @@ __global__ void k_settle(Solve *solves, int first) {
   if (threadIdx.x == 0) D.ctl[3] = 0;
 }
 
+// A handle of several solves: workgroup b takes item b of the ghost work list (ghost_list).
 template <typename T>
-__global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves) {
+__global__ __launch_bounds__(BLOCK) void k_ghost(const Solve *solves, const int4 *gwork) {
   __shared__ double red[BLOCK];
-  ghost_cells<T>(solves[blockIdx.y], red);
+  const int4 it = gwork[blockIdx.x];
+  ghost_cells<T>(solves[it.x], red, it.z, it.y);
 }
 
 // A handle of one solve: the descriptor travels in the kernel arguments.
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_ghost1(const Solve one) {
   __shared__ double red[BLOCK];
-  ghost_cells<T>(one, red);
+  // grid (cell blocks + 1, 1, tracers): the last block of each layer is the correction total's
+  ghost_cells<T>(one, red, blockIdx.x == gridDim.x - 1 ? -1 : (int)blockIdx.x, (int)blockIdx.z);
 }
 
 // Fold the correction totals (interior) and the FP32 state offsets (every cell: a ghost value is
@@ -778,10 +780,10 @@ int step_ghosts(csadv_t *h, hipStream_t st) {
   const bool one = ns == 1;
   if (h->dtype == CSADV_F64) {
     if (one) hipLaunchKernelGGL(k_ghost1<double>, grid, dim3(BLOCK), 0, st, h->host[0]);
-    else hipLaunchKernelGGL(k_ghost<double>, grid, dim3(BLOCK), 0, st, h->dev);
+    else hipLaunchKernelGGL(k_ghost<double>, dim3(h->ngwork), dim3(BLOCK), 0, st, h->dev, h->gwork);
   } else {
     if (one) hipLaunchKernelGGL(k_ghost1<float>, grid, dim3(BLOCK), 0, st, h->host[0]);
-    else hipLaunchKernelGGL(k_ghost<float>, grid, dim3(BLOCK), 0, st, h->dev);
+    else hipLaunchKernelGGL(k_ghost<float>, dim3(h->ngwork), dim3(BLOCK), 0, st, h->dev, h->gwork);
   }
   HIPCHK(h, hipGetLastError());
   return CSADV_OK;
@@ -912,6 +914,16 @@ int csadv_update(csadv_t *h, const csadv_solve_desc *solves, int32_t nsolve, voi
   for (int s = 0; s < nsolve; ++s)
     h->wind1 = h->wind1 && solves[s].unsteady && solves[s].dp == 1 && solves[s].nbasis == 1;
   HIPCHK(h, hipMemcpyAsync(h->work, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice, st));
+  if (nsolve > 1) {                  // the ghost fill of a multi-solve handle reads its own list
+    const std::vector<Item> gw = ghost_list(h->host);
+    if ((int)gw.size() != h->ngwork) {
+      if (h->gwork) (void)hipFree(h->gwork);
+      h->gwork = nullptr;
+      HIPCHK(h, hipMalloc(&h->gwork, sizeof(int4) * gw.size()));
+      h->ngwork = (int)gw.size();
+    }
+    HIPCHK(h, hipMemcpyAsync(h->gwork, gw.data(), sizeof(int4) * gw.size(), hipMemcpyHostToDevice, st));
+  }
   HIPCHK(h, hipMemcpyAsync(h->dev, h->host.data(), sizeof(Solve) * nsolve, hipMemcpyHostToDevice, st));
   HIPCHK(h, hipMemcpyAsync(h->qsets, sets.data(), sizeof(FieldSet) * nsolve, hipMemcpyHostToDevice, st));
   // steps already completed by each solve (a handle may be built over solves that have run)
@@ -1351,6 +1363,7 @@ void csadv_destroy(csadv_t *h) {
   if (!h) return;
   if (h->dev) (void)hipFree(h->dev);
   if (h->work) (void)hipFree(h->work);
+  if (h->gwork) (void)hipFree(h->gwork);
   if (h->qsets) (void)hipFree(h->qsets);
   if (h->wsets) (void)hipFree(h->wsets);
   if (h->red) (void)hipFree(h->red);

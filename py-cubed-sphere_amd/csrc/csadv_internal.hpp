@@ -18,9 +18,11 @@ struct csadv_s {
   std::vector<Solve> host;
   Solve *dev = nullptr;           // device descriptors (caller-visible scratch is not used)
   int4 *work = nullptr;           // step work items
+  int4 *gwork = nullptr;          // ghost-fill work items of a multi-solve handle (ghost_list)
   FieldSet *qsets = nullptr;      // ghost-fill sets for the tracers
   FieldSet *wsets = nullptr;      // ghost-fill sets for the lat-lon centre winds
   int nwork = 0;
+  int ngwork = 0;
   int tile = 0;
   int dtype = 0;
   int recon = 3;

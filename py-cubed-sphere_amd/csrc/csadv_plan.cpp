@@ -120,6 +120,21 @@ std::vector<Item> work_list(const std::vector<Solve> &solves, int group) {
   return work;
 }
 
+std::vector<Item> ghost_list(const std::vector<Solve> &solves) {
+  int maxtr = 0;
+  for (auto &S : solves) maxtr = std::max(maxtr, S.d.ntr);
+  std::vector<Item> g;
+  for (int tr = 0; tr < maxtr; ++tr)
+    for (int s = 0; s < (int)solves.size(); ++s) {
+      const Solve &S = solves[s];
+      if (tr >= S.d.ntr) continue;
+      const int blocks = (S.d.n_strip + S.d.n_corner + BLOCK - 1) / BLOCK;
+      for (int b = 0; b < blocks; ++b) g.push_back(Item{s, tr, b, 0});
+      if (S.d.mf == 3) g.push_back(Item{s, tr, -1, 0});
+    }
+  return g;
+}
+
 int partials_per_tracer(int N, int tile) {
   if (tile < 0 || tile >= kNumTiles || N < 1) return CSADV_E_ARG;
   int sw, ns, sl, nsg;

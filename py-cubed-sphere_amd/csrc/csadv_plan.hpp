@@ -133,6 +133,13 @@ Solve derive(const csadv_solve_desc &d, int tile);
 // step kernel marches two).
 std::vector<Item> work_list(const std::vector<Solve> &solves, int group = 1);
 
+// The per-step ghost fill's work list of a multi-solve handle: (solve, tracer, cell block) for
+// every block of BLOCK ghost cells of every tracer, each tracer's blocks followed by its
+// correction-total block (cell block -1, MF-PR only), tracer layer by tracer layer (all solves'
+// blocks of tracer 0 first) as in a dense (blocks, solves, tracers) grid, without that grid's
+// blocks past a solve's cells or tracers.
+std::vector<Item> ghost_list(const std::vector<Solve> &solves);
+
 // Sizes of the public entry points (see include/csadv.h).
 int partials_per_tracer(int N, int tile);
 int work_bytes(int N, int dtype, int64_t *bytes);

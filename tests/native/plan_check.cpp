@@ -178,6 +178,28 @@ int main() {
     }
     CHECK(real == expect, "work list holds %lld of %lld items", (long long)real, (long long)expect);
     ++cases;
+    // the ghost fill's list: every cell block of every tracer once, the correction-total block
+    // (-1) of a tracer exactly with MF-PR, tracer layers in order, nothing past a solve
+    if (trial % 3 == 0)
+      for (auto &S : solves) S.d.mf = 1 + (int)(rng() % 3);
+    const std::vector<Item> gw = ghost_list(solves);
+    std::set<std::tuple<int, int, int>> gseen;
+    int64_t gexpect = 0;
+    for (auto &S : solves)
+      gexpect += (int64_t)S.d.ntr * ((S.d.n_strip + S.d.n_corner + BLOCK - 1) / BLOCK + (S.d.mf == 3 ? 1 : 0));
+    CHECK((int64_t)gw.size() == gexpect, "ghost list holds %zu of %lld items", gw.size(), (long long)gexpect);
+    int last_tr = 0;
+    for (const Item &it : gw) {
+      CHECK(it.x >= 0 && it.x < nsolve, "ghost item solve");
+      const Solve &S = solves[it.x];
+      const int blocks = (S.d.n_strip + S.d.n_corner + BLOCK - 1) / BLOCK;
+      CHECK(it.y >= 0 && it.y < S.d.ntr, "ghost item tracer");
+      CHECK(it.z >= -1 && it.z < blocks && (it.z >= 0 || S.d.mf == 3), "ghost item block %d", it.z);
+      CHECK(it.y >= last_tr, "ghost items not tracer layer by layer");
+      last_tr = it.y;
+      CHECK(gseen.insert({it.x, it.y, it.z}).second, "ghost item listed twice");
+    }
+    ++cases;
   }
   // ---- general step layout: arrays disjoint, in bounds, FP64 tail aligned
   for (int N : {8, 9, 17, 48, 384, 768, 8192}) {
